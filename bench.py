@@ -1,0 +1,186 @@
+"""Benchmark of the APSP evaluator on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c5]
+
+A step is one exact evaluation (diameter + distance sum S) of the workload
+graph(s), with the adjacency already resident in HBM: one replay of a captured
+hipGraph (adjacency transposition + tile BFS + per-graph combine + result copy).
+
+Prints ONE JSON line.  `value` is evaluations per second; the roofline block
+prices the tile-BFS kernel with the canonical algorithmic bytes of the
+reference's single-tile bit-parallel BFS,
+    bytes = L * n * (4*d + 8*w*(d+2)),  w = ceil(B/64),  L = diameter,
+against MI355X's 8 TB/s HBM peak; the CPU baseline is the C/OpenMP port of the
+reference's matrix_op (oracle/) timed on this host on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBPS = 8000.0
+METRIC = "APSP evals/sec + achieved GB/s; SA time-to-ASPL, n=65536 d=16 on 1 GPU"
+
+WORKLOADS = {
+    # name: (description, builder)
+    "c5": "apsp n=65536 d=16 g=1 (random regular, seed 5)",
+    "c2": "apsp n=4096 d=16 g=4 (rotationally symmetric, seed 11)",
+    "c1": "apsp n=256 d=8 g=1 (random regular, seed 1)",
+    "c4": "apsp grid 100x100 d=4 r=2 (seed 1)",
+    "c3": "batch of 512 graphs n~U[1024,16384] d in {8,12,16,24,32} (one 2-opt each)",
+}
+
+
+def build_workload(name):
+    """Returns a list of dict(adj, based_nodes, groups, relabel, symmetric)."""
+    from graphgolf_amd import graphs as G
+
+    if name == "c5":
+        adj = G.edges_to_adj(65536, 16, G.random_regular(65536, 16, 5))
+        return [dict(adj=adj, based_nodes=65536, groups=1, symmetric=True)]
+    if name == "c2":
+        adj = G.edges_to_adj(4096, 16, G.random_symmetric(4096, 16, 4, 11))
+        return [dict(adj=adj, based_nodes=1024, groups=4, symmetric=True)]
+    if name == "c1":
+        adj = G.edges_to_adj(256, 8, G.random_regular(256, 8, 1))
+        return [dict(adj=adj, based_nodes=256, groups=1, symmetric=True)]
+    if name == "c4":
+        adj = G.edges_to_adj(10000, 4, G.random_grid(100, 100, 4, 2, 1))
+        return [dict(adj=adj, based_nodes=10000, groups=1, symmetric=True,
+                     relabel=np.arange(10000, dtype=np.int32))]
+    if name == "c3":
+        sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+        from cases import c3_shapes, two_opt  # seeded shapes shared with the tests
+
+        out = []
+        for (n, d, s) in c3_shapes(512, 7):
+            e = two_opt(G.random_regular(n, d, s), n, np.random.default_rng(s), 1)
+            out.append(dict(adj=G.edges_to_adj(n, d, e), based_nodes=n, groups=1, symmetric=True))
+        return out
+    raise SystemExit(f"unknown workload {name}")
+
+
+def algorithmic_bytes(n, d, B, diameter):
+    w = (B + 63) // 64
+    return diameter * n * (4 * d + 8 * w * (d + 2))
+
+
+def cpu_baseline(items, budget_s):
+    """The oracle's C/OpenMP port of matrix_op on a bounded sample."""
+    import oracle
+
+    it = items[0]
+    adj = it["adj"]
+    n = adj.shape[0]
+    t0 = time.perf_counter()
+    runs = 0
+    times = []
+    while True:
+        t = time.perf_counter()
+        if it.get("relabel") is not None:
+            oracle.grid_evaluation(adj, it["groups"], it["based_nodes"], 100, 100, it["relabel"])
+        else:
+            oracle.matrix_op(adj, it["based_nodes"], it["groups"], 0)
+        times.append(time.perf_counter() - t)
+        runs += 1
+        if time.perf_counter() - t0 > budget_s or runs >= 50:
+            break
+    med = float(np.median(times))
+    return dict(value=(1.0 / med) * len(items) / len(items), unit="evals/s",
+                cores=int(oracle.threads()), kind="port",
+                sample=f"{runs} evaluations of the first workload graph (n={n}), median "
+                       f"{med:.4f} s; OpenMP threads={oracle.threads()}")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="c5", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-budget", type=float, default=15.0,
+                    help="seconds of CPU time for the CPU baseline sample (0 = skip)")
+    ap.add_argument("--word-bits", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build evaluates on a single MI355X "
+              "(multi-GPU sharding is described in DESIGN.md, not run).", file=sys.stderr)
+        raise SystemExit(2)
+
+    from graphgolf_amd import evaluation as E
+
+    items = build_workload(args.workload)
+    E.init(0)
+    if args.word_bits:
+        E.set_word_bits(args.word_bits)
+    plan = E.Plan([dict(nodes=it["adj"].shape[0], degree=it["adj"].shape[1],
+                        based_nodes=it["based_nodes"], groups=it["groups"],
+                        relabel=it.get("relabel"), symmetric=it["symmetric"]) for it in items])
+    for i, it in enumerate(items):
+        plan.set_adj(i, it["adj"])  # resident in HBM before the timed region
+    res = None
+    for _ in range(args.warmup):
+        res = plan.run()
+    E.synchronize()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.launch()
+        res = plan.wait()
+        kernel_ms.append(plan.kernel_ms())
+    E.synchronize()
+    elapsed = time.perf_counter() - t0
+
+    ms_per_step = 1000.0 * elapsed / args.steps
+    evals_per_s = len(items) * args.steps / elapsed
+    total_bytes = sum(algorithmic_bytes(it["adj"].shape[0], it["adj"].shape[1], it["based_nodes"],
+                                        r.diameter) for it, r in zip(items, res))
+    k_ms = float(np.mean(kernel_ms))
+    achieved = total_bytes / (k_ms * 1e-3) / 1e9
+    r0 = res[0]
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    cpu = None
+    if args.cpu_budget > 0:
+        cpu = cpu_baseline(items, args.cpu_budget)
+    line = {
+        "metric": METRIC,
+        "value": round(evals_per_s, 3),
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": f"u{r0.word_bits}",
+        "data": "synthetic",
+        "config": {"workload": WORKLOADS[args.workload], "graphs": len(items),
+                   "nodes": int(items[0]["adj"].shape[0]), "degree": int(items[0]["adj"].shape[1]),
+                   "groups": int(items[0]["groups"]), "word_bits": r0.word_bits,
+                   "diameter": r0.diameter, "sum": r0.sum, "aspl": r0.aspl},
+        "achieved_GBps": round(achieved, 1),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                     "traffic": traffic, "kernel_ms": round(k_ms, 4),
+                     "algorithmic_bytes": int(total_bytes)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    plan.close()
+
+
+if __name__ == "__main__":
+    main()

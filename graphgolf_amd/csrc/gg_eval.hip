@@ -1,0 +1,1231 @@
+// gg_eval.hip -- gfx950 bit-parallel APSP evaluator (kernels + host engine).
+//
+// What it computes (bit-exact with the reference's evaluation()):
+//   general/evaluation.c:153-263 (matrix_op), :265-373 (tiled variant) and
+//   grid/evaluation.c:122-190 (rotate_hash relabelling).  For every source
+//   tile the reference runs a level-synchronous BFS over bitsets
+//   ("bit s of row v = source s reaches v"), counts reached (source, vertex)
+//   pairs per level, and accumulates the unreached count until every pair is
+//   reached.  S and the diameter are integer functions of those per-level
+//   counts, so any algorithm that produces the same reach sets per level
+//   produces identical outputs.
+//
+// How it is laid out on MI355X:
+//   * One workgroup owns one (graph, source tile) unit and runs ALL of its
+//     BFS levels with the tile's reach bitsets resident in LDS: R[v] is one
+//     W-bit word per vertex (W = 8/16/32/64 sources).  n=65536 with W=16
+//     is 128 KiB, inside gfx950's 160 KiB LDS, so the level loop never
+//     touches HBM except for the adjacency stream.
+//   * The adjacency is converted once per call into a transposed, uint16,
+//     degree-padded table adjT[j][v] (2 B per entry, coalesced across lanes
+//     that own consecutive vertices); it is read from L2 by every tile.
+//   * Direction-optimising levels.  While the frontier of the tile is small
+//     it is kept as an LDS queue of (vertex, new bits) and pushed to the
+//     neighbours with LDS atomic ORs (top-down, only for symmetric
+//     adjacency, which the prep kernel verifies).  Once the queue overflows
+//     the tile switches to pull levels (bottom-up): every vertex ORs its
+//     neighbours' words, stopping as soon as its own word is full; the new
+//     words are staged in registers and written back after a barrier.
+//   * Counts: wave64 shuffle reductions, two barriers per level; per-unit
+//     integer partial sums combined per graph by a tiny kernel.
+//   * No MFMA: the work is integer bit logic and popcounts.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cinttypes>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "gg_eval.h"
+#include "gg_host.h"
+#include "gg_internal.h"
+
+// ---------------------------------------------------------------------------
+// Device-side descriptors.
+
+struct GraphDev {
+  const uint16_t *adjT;  // [dp][n] transposed, padded with the row's own id
+  int32_t *flags;        // [0] = bad adjacency entries, [1] = asymmetric
+  int32_t n, dp, split, c;
+  int32_t push_ok, groups;
+  int32_t hist_maxd;     // >0: accumulate per-vertex distance histogram
+  int32_t pad;
+  int32_t *hist;         // [n][hist_maxd+1] (histogram mode only)
+};
+
+struct UnitDesc {
+  int32_t graph;
+  int32_t first_row;
+  int32_t l;
+  int32_t centre;
+};
+
+struct UnitOut {
+  uint64_t acc_lo, acc_hi;
+  int32_t done_level, last_change, disconnected, error;
+};
+
+struct GraphMeta {
+  int32_t unit_begin, unit_count, n, groups;
+};
+
+struct GraphOut {
+  uint64_t sum;
+  int32_t connected, diameter, max_ecc, error;
+};
+
+// ---------------------------------------------------------------------------
+// Word helpers.
+
+template <typename W> struct WordBits { static constexpr int value = 8 * sizeof(W); };
+
+// Pointer into the global address space.
+template <typename T> using gptr = __attribute__((address_space(1))) T *;
+
+template <typename W> __device__ __forceinline__ int wpopc(W x) {
+  if constexpr (sizeof(W) == 8) return __popcll((unsigned long long)x);
+  else return __popc((unsigned)x);
+}
+
+// LDS atomic OR of `f` into element x of R; returns the element's old value.
+template <typename W> __device__ __forceinline__ W lds_or(W *R, uint32_t x, W f) {
+  if constexpr (sizeof(W) == 8) {
+    return (W)atomicOr((unsigned long long *)(R + x), (unsigned long long)f);
+  } else if constexpr (sizeof(W) == 4) {
+    return (W)atomicOr((unsigned *)(R + x), (unsigned)f);
+  } else if constexpr (sizeof(W) == 2) {
+    unsigned *w = (unsigned *)R + (x >> 1);
+    unsigned sh = (x & 1u) * 16u;
+    return (W)(atomicOr(w, (unsigned)f << sh) >> sh);
+  } else {
+    unsigned *w = (unsigned *)R + (x >> 2);
+    unsigned sh = (x & 3u) * 8u;
+    return (W)(atomicOr(w, (unsigned)f << sh) >> sh);
+  }
+}
+
+// Register staging of VPT words per thread, packed into 32-bit registers.
+template <typename W, int VPT> struct StageRegs {
+  static constexpr int value = (VPT * (int)sizeof(W) + 3) / 4;
+};
+
+template <typename W> __device__ __forceinline__ void stage_put(uint32_t *st, int r, W val) {
+  if constexpr (sizeof(W) == 8) {
+    st[2 * r] = (uint32_t)val;
+    st[2 * r + 1] = (uint32_t)((uint64_t)val >> 32);
+  } else if constexpr (sizeof(W) == 4) {
+    st[r] = (uint32_t)val;
+  } else if constexpr (sizeof(W) == 2) {
+    if (r & 1) st[r >> 1] |= (uint32_t)val << 16;
+    else st[r >> 1] = (uint32_t)val;
+  } else {
+    if (r & 3) st[r >> 2] |= (uint32_t)val << ((r & 3) * 8);
+    else st[r >> 2] = (uint32_t)val;
+  }
+}
+
+template <typename W> __device__ __forceinline__ W stage_get(const uint32_t *st, int r) {
+  if constexpr (sizeof(W) == 8) return (W)(((uint64_t)st[2 * r + 1] << 32) | st[2 * r]);
+  else if constexpr (sizeof(W) == 4) return (W)st[r];
+  else if constexpr (sizeof(W) == 2) return (W)(st[r >> 1] >> ((r & 1) * 16));
+  else return (W)(st[r >> 2] >> ((r & 3) * 8));
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// Adjacency preparation: row-major int32 [n][d] (+ optional relabel) ->
+// transposed uint16 [dp][n].  Invalid entries are replaced by the row's own
+// id (memory-safe) and counted in flags[0]; flags[1] counts asymmetric
+// entries when check_sym is set.
+
+__global__ void gg_prep_kernel(const int32_t *__restrict__ adj, const int32_t *__restrict__ relabel,
+                               int n, int d, int dp, uint16_t *__restrict__ adjT,
+                               int32_t *__restrict__ flags, int check_sym) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int ri = relabel ? relabel[i] : i;
+  const int32_t *row = adj + (size_t)i * d;
+  int bad = 0, asym = 0;
+  for (int j = 0; j < d; ++j) {
+    int u = row[j];
+    int ru;
+    if (u < 0 || u >= n) {
+      ++bad;
+      ru = ri;
+    } else {
+      ru = relabel ? relabel[u] : u;
+      if (check_sym) {
+        int c1 = 0, c2 = 0;
+        const int32_t *urow = adj + (size_t)u * d;
+        for (int k = 0; k < d; ++k) {
+          c1 += (row[k] == u);
+          c2 += (urow[k] == i);
+        }
+        asym += (c1 != c2);
+      }
+    }
+    adjT[(size_t)j * n + ri] = (uint16_t)ru;
+  }
+  for (int j = d; j < dp; ++j) adjT[(size_t)j * n + ri] = (uint16_t)ri;
+  if (bad) atomicAdd(&flags[0], bad);
+  if (asym) atomicAdd(&flags[1], asym);
+}
+
+// ---------------------------------------------------------------------------
+// The tile BFS.  One workgroup = one UnitDesc.  LDS layout (dynamic):
+//   R[n_cap] (W) | qb[2][qcap] (W) | qv[2][qcap] (u16) | red[2][16][4] | qcount[2]
+
+template <typename W, int BLOCK, int VPT, bool HIST>
+__global__ void __launch_bounds__(BLOCK)
+gg_tile_bfs_kernel(const GraphDev *__restrict__ graphs, const UnitDesc *__restrict__ units,
+                   UnitOut *__restrict__ outs, int unit_base, int n_cap, int qcap) {
+  constexpr int BITS = WordBits<W>::value;
+  constexpr int NWAVES = BLOCK / 64;
+  extern __shared__ __align__(16) unsigned char smem[];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int uidx = unit_base + blockIdx.x;
+  const UnitDesc U = units[uidx];
+  const GraphDev G = graphs[U.graph];
+  const int n = G.n, dp = G.dp, split = G.split;
+  const int c = n - split;
+  // The descriptor's pointer is generic; say it is global so the gathers
+  // compile to global_load (not flat_load).
+  const gptr<const uint16_t> adjT = (gptr<const uint16_t>)G.adjT;
+
+  size_t off = 0;
+  W *R = (W *)(smem + off);
+  off += ((size_t)n_cap * sizeof(W) + 15) & ~(size_t)15;
+  W *qb0 = (W *)(smem + off);
+  W *qb1 = qb0 + qcap;
+  off += ((size_t)2 * qcap * sizeof(W) + 15) & ~(size_t)15;
+  uint16_t *qv0 = (uint16_t *)(smem + off);
+  uint16_t *qv1 = qv0 + qcap;
+  off += ((size_t)2 * qcap * sizeof(uint16_t) + 15) & ~(size_t)15;
+  uint32_t *red = (uint32_t *)(smem + off);  // [2][NWAVES][4]
+  off += 2 * 16 * 4 * sizeof(uint32_t);
+  uint32_t *qcount = (uint32_t *)(smem + off);  // [2]
+
+  const int32_t bad = G.flags[0];
+  if (bad != 0) {  // uniform: adjacency had out-of-range entries
+    if (tid == 0) {
+      UnitOut o = {0, 0, n + 1, 0, 1, 1};
+      outs[uidx] = o;
+    }
+    return;
+  }
+  const bool push_ok = G.push_ok && G.flags[1] == 0;
+
+  const int l = U.l, first = U.first_row;
+  const bool centre = U.centre != 0;
+  const W full = (l >= BITS) ? (W)~(W)0 : (W)(((W)1 << l) - (W)1);
+
+  for (int v = tid; v < n; v += BLOCK) R[v] = (W)0;
+  if (tid == 0) {
+    qcount[0] = (uint32_t)l;
+    qcount[1] = 0;
+  }
+  __syncthreads();
+  if (tid < l) {
+    W b = (W)((W)1 << tid);
+    R[first + tid] = b;
+    qb0[tid] = b;
+    qv0[tid] = (uint16_t)(first + tid);
+    if constexpr (HIST) atomicAdd(&G.hist[(size_t)(first + tid) * (G.hist_maxd + 1)], 1);
+  }
+  __syncthreads();
+
+  const uint32_t lo_full = (uint32_t)split * (uint32_t)l;
+  const uint32_t hi_full = (uint32_t)c * (uint32_t)l;
+  int src_lo = split - first;
+  src_lo = src_lo < 0 ? 0 : (src_lo > l ? l : src_lo);
+  uint32_t tot_lo = (uint32_t)src_lo, tot_hi = (uint32_t)(l - src_lo);
+
+  uint64_t acc_lo = 0, acc_hi = 0;
+  int level = 0, done_level = n + 1, last_change = 0, disconnected = 0;
+  bool qvalid = push_ok && l <= qcap;
+  uint32_t qn = (uint32_t)l;
+  int cur = 0;
+
+  while (true) {
+    ++level;
+    const bool do_push = qvalid && (qn * 4u <= (uint32_t)n);
+    uint32_t p_lo = 0, p_hi = 0, p_chg = 0;
+    // Staged pull results, packed into 32-bit registers (fully unrolled, so
+    // every index below is a compile-time constant).
+    uint32_t st[StageRegs<W, VPT>::value];
+    if (tid == 0) qcount[cur] = 0;  // this level reads qn (register); slot is next-next queue
+
+    if (do_push) {
+      // Top-down: push each frontier entry's new bits to its neighbours.
+      const W *qb_c = cur ? qb1 : qb0;
+      const uint16_t *qv_c = cur ? qv1 : qv0;
+      W *qb_n = cur ? qb0 : qb1;
+      uint16_t *qv_n = cur ? qv0 : qv1;
+      uint32_t *qc_n = &qcount[cur ^ 1];
+      for (uint32_t e = tid; e < qn; e += BLOCK) {
+        const uint32_t u = qv_c[e];
+        const W f = qb_c[e];
+        for (int j = 0; j < dp; j += 4) {
+          uint32_t x[4];
+#pragma unroll
+          for (int t = 0; t < 4; ++t) x[t] = adjT[(size_t)(j + t) * n + u];
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            const W old = lds_or<W>(R, x[t], f);
+            const W nb = (W)(f & (W)~old);
+            if (nb) {
+              const uint32_t pc = (uint32_t)wpopc<W>(nb);
+              if ((int)x[t] < split) p_lo += pc;
+              else p_hi += pc;
+              if constexpr (HIST)
+                atomicAdd(&G.hist[(size_t)x[t] * (G.hist_maxd + 1) + min(level, G.hist_maxd)],
+                          (int)pc);
+              const uint32_t slot = atomicAdd(qc_n, 1u);
+              if (slot < (uint32_t)qcap) {
+                qb_n[slot] = nb;
+                qv_n[slot] = (uint16_t)x[t];
+              }
+            }
+          }
+        }
+      }
+    } else {
+      // Bottom-up: every vertex gathers its neighbours' words (early exit on full).
+      gptr<const uint16_t> a = adjT + tid;
+#pragma unroll
+      for (int r = 0; r < VPT; ++r) {
+        const int v = r * BLOCK + tid;
+        W val = (W)0;
+        if (v < n) {
+          const W old = R[v];
+          val = old;
+          if (val != full) {
+            gptr<const uint16_t> p = a;
+            for (int j = 0; j < dp; j += 4) {
+              const uint32_t i0 = p[0];
+              const uint32_t i1 = p[n];
+              const uint32_t i2 = p[2 * n];
+              const uint32_t i3 = p[3 * n];
+              val = (W)(val | R[i0] | R[i1] | R[i2] | R[i3]);
+              if (val == full) break;
+              p += 4 * n;
+            }
+          }
+          const uint32_t pc = (uint32_t)wpopc<W>(val);
+          if (v < split) p_lo += pc;
+          else p_hi += pc;
+          if (val != old) {
+            p_chg = 1;
+            if constexpr (HIST)
+              atomicAdd(&G.hist[(size_t)v * (G.hist_maxd + 1) + min(level, G.hist_maxd)],
+                        wpopc<W>((W)(val & (W)~old)));
+          }
+        }
+        stage_put<W>(st, r, val);
+        a += BLOCK;
+        // Keep one running pointer instead of VPT precomputed addresses.
+        asm volatile("" : "+v"(a));
+      }
+    }
+
+    // Block reduction of (lo, hi, changed).
+    p_lo = wave_sum(p_lo);
+    p_hi = wave_sum(p_hi);
+    p_chg = wave_sum(p_chg);
+    uint32_t *rb = red + (level & 1) * 16 * 4;
+    if (lane == 0) {
+      rb[wave * 4 + 0] = p_lo;
+      rb[wave * 4 + 1] = p_hi;
+      rb[wave * 4 + 2] = p_chg;
+    }
+    __syncthreads();  // B1: all reads of R for this level are done
+    if (!do_push) {
+#pragma unroll
+      for (int r = 0; r < VPT; ++r) {
+        const int v = r * BLOCK + tid;
+        if (v < n) R[v] = stage_get<W>(st, r);
+      }
+    }
+    uint32_t L_lo = 0, L_hi = 0, L_chg = 0;
+#pragma unroll
+    for (int w = 0; w < NWAVES; ++w) {
+      L_lo += rb[w * 4 + 0];
+      L_hi += rb[w * 4 + 1];
+      L_chg += rb[w * 4 + 2];
+    }
+    const uint32_t qn_next = qcount[cur ^ 1];
+    __syncthreads();  // B2: R write-back visible, counters read
+
+    bool chg;
+    if (do_push) {
+      tot_lo += L_lo;
+      tot_hi += L_hi;
+      chg = (L_lo + L_hi) != 0;
+      qvalid = qn_next <= (uint32_t)qcap;
+      qn = qn_next;
+      cur ^= 1;
+    } else {
+      tot_lo = L_lo;
+      tot_hi = L_hi;
+      chg = L_chg != 0;
+      qvalid = false;
+    }
+    if (chg) last_change = level;
+    const bool done = centre ? (tot_hi == hi_full) : (tot_lo + tot_hi == lo_full + hi_full);
+    if (done) {
+      done_level = level;
+      break;
+    }
+    if (!chg || level >= n) {
+      disconnected = 1;
+      done_level = n + 1;
+      break;
+    }
+    if (!centre) acc_lo += (uint64_t)(lo_full - tot_lo);
+    acc_hi += (uint64_t)(hi_full - tot_hi);
+  }
+
+  if (tid == 0) {
+    UnitOut o;
+    o.acc_lo = acc_lo;
+    o.acc_hi = acc_hi;
+    o.done_level = done_level;
+    o.last_change = last_change;
+    o.disconnected = disconnected;
+    o.error = 0;
+    outs[uidx] = o;
+  }
+}
+
+// Per-graph combination of the unit partial sums
+// (general/evaluation.c:199-200, :241, :247-249 as integers).
+__global__ void gg_combine_kernel(const GraphMeta *__restrict__ meta, const UnitDesc *__restrict__ units,
+                                  const UnitOut *__restrict__ outs, GraphOut *__restrict__ res,
+                                  int ngraphs) {
+  const int gidx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gidx >= ngraphs) return;
+  const GraphMeta m = meta[gidx];
+  uint64_t lo = 0, hi = 0, chi = 0;
+  int diam = 1, ecc = 0, disc = 0, err = 0;
+  for (int k = 0; k < m.unit_count; ++k) {
+    const int u = m.unit_begin + k;
+    const UnitOut o = outs[u];
+    if (units[u].centre) chi += o.acc_hi;
+    else {
+      lo += o.acc_lo;
+      hi += o.acc_hi;
+    }
+    diam = max(diam, o.done_level);
+    ecc = max(ecc, o.last_change);
+    disc |= o.disconnected;
+    err |= o.error;
+  }
+  GraphOut r;
+  const uint64_t g = (uint64_t)m.groups, n = (uint64_t)m.n;
+  r.sum = lo * g + hi * 2ull * g + chi + n * (n - 1ull);
+  r.connected = disc ? 0 : 1;
+  r.diameter = disc ? m.n + 1 : diam;
+  r.max_ecc = ecc;
+  r.error = err;
+  res[gidx] = r;
+}
+
+// A tiny fill kernel (captured graphs must not record memset nodes).
+__global__ void gg_clear_i32_kernel(int32_t *p, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Host engine.
+
+namespace gg {
+
+static thread_local char g_err[512] = "";
+
+int set_error(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+  return code;
+}
+const char *last_error() { return g_err; }
+
+#define GG_HIP(call)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return set_error(GG_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                       __FILE__, __LINE__);                                           \
+  } while (0)
+
+// Dense LDS limit per workgroup on gfx950.
+static constexpr int kLdsBytes = 160 * 1024;
+static constexpr int kMaxNodes = 65536;
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t cap = 0;
+  int ensure(size_t bytes) {
+    if (bytes <= cap) return GG_OK;
+    if (p) hipFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t want = std::max(bytes, (size_t)256);
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess)
+      return set_error(GG_ERR_HIP, "hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
+    cap = want;
+    return GG_OK;
+  }
+  void release() {
+    if (p) hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+struct PinBuf {
+  void *p = nullptr;
+  size_t cap = 0;
+  int ensure(size_t bytes) {
+    if (bytes <= cap) return GG_OK;
+    if (p) hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t want = std::max(bytes, (size_t)256);
+    hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+    if (e != hipSuccess)
+      return set_error(GG_ERR_HIP, "hipHostMalloc(%zu) failed: %s", want, hipGetErrorString(e));
+    cap = want;
+    return GG_OK;
+  }
+  void release() {
+    if (p) hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+// Kernel instance key: word width, block size, vertices per thread, histogram.
+struct KernelKey {
+  int bits, block, vpt, hist;
+  bool operator<(const KernelKey &o) const {
+    return std::tie(bits, block, vpt, hist) < std::tie(o.bits, o.block, o.vpt, o.hist);
+  }
+};
+
+typedef void (*TileKernel)(const GraphDev *, const UnitDesc *, UnitOut *, int, int, int);
+
+template <typename W, int B, int V, bool H>
+static TileKernel kptr() {
+  return (TileKernel)&gg_tile_bfs_kernel<W, B, V, H>;
+}
+
+template <typename W, bool H>
+static TileKernel pick_vpt(int block, int vpt) {
+  if (block == 256) {
+    switch (vpt) {
+      case 1: return kptr<W, 256, 1, H>();
+      case 2: return kptr<W, 256, 2, H>();
+      case 4: return kptr<W, 256, 4, H>();
+      case 8: return kptr<W, 256, 8, H>();
+    }
+  } else {
+    switch (vpt) {
+      case 4: return kptr<W, 1024, 4, H>();
+      case 8: return kptr<W, 1024, 8, H>();
+      case 16: return kptr<W, 1024, 16, H>();
+      case 32: return kptr<W, 1024, 32, H>();
+      case 64:
+        if constexpr (sizeof(W) <= 4) return kptr<W, 1024, 64, H>();
+        break;
+    }
+  }
+  return nullptr;
+}
+
+static TileKernel kernel_for(const KernelKey &k) {
+  const bool h = k.hist != 0;
+  switch (k.bits) {
+    case 8: return h ? pick_vpt<uint8_t, true>(k.block, k.vpt) : pick_vpt<uint8_t, false>(k.block, k.vpt);
+    case 16: return h ? pick_vpt<uint16_t, true>(k.block, k.vpt) : pick_vpt<uint16_t, false>(k.block, k.vpt);
+    case 32: return h ? pick_vpt<uint32_t, true>(k.block, k.vpt) : pick_vpt<uint32_t, false>(k.block, k.vpt);
+    case 64: return h ? pick_vpt<uint64_t, true>(k.block, k.vpt) : pick_vpt<uint64_t, false>(k.block, k.vpt);
+  }
+  return nullptr;
+}
+
+static void block_vpt_for(int n, int *block, int *vpt) {
+  if (n <= 2048) {
+    *block = 256;
+    int v = (n + 255) / 256;
+    *vpt = v <= 1 ? 1 : v <= 2 ? 2 : v <= 4 ? 4 : 8;
+  } else {
+    *block = 1024;
+    int v = (n + 1023) / 1024;
+    *vpt = v <= 4 ? 4 : v <= 8 ? 8 : v <= 16 ? 16 : v <= 32 ? 32 : 64;
+  }
+}
+
+static size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
+
+static size_t lds_bytes_for(int bits, int n_cap, int qcap) {
+  size_t wb = bits / 8;
+  return align16((size_t)n_cap * wb) + align16((size_t)2 * qcap * wb) +
+         align16((size_t)2 * qcap * 2) + 2 * 16 * 4 * 4 + 16;
+}
+
+// The largest queue that fits the LDS budget, capped at what push levels
+// can use (a push level needs qn*4 <= n).
+static int qcap_for(int bits, int n, size_t budget) {
+  size_t base = lds_bytes_for(bits, n, 0);
+  if (base >= budget) return -1;
+  size_t per = 2 * (size_t)(bits / 8) + 2 * 2;
+  size_t fit = (budget - base - 64) / per;  // slack for the alignment rounding
+  int want = std::max(64, n / 4 + 64);
+  return (int)std::min(fit, (size_t)want);
+}
+
+// Tuning table: (n bucket, d) -> word bits.
+struct TuneKey {
+  int n, d, b, g, batch;
+  bool operator<(const TuneKey &o) const {
+    return std::tie(n, d, b, g, batch) < std::tie(o.n, o.d, o.b, o.g, o.batch);
+  }
+};
+
+struct Engine {
+  bool ready = false;
+  int device = 0;
+  int cus = 256;
+  hipStream_t stream = nullptr;
+  int forced_bits = 0;
+  std::map<TuneKey, int> tune;
+};
+
+static Engine E;
+
+Engine &engine() { return E; }
+
+int init(int device) {
+  if (E.ready) return GG_OK;
+  if (device < 0) {
+    const char *s = getenv("GG_DEVICE");
+    device = s ? atoi(s) : 0;
+  }
+  int cnt = 0;
+  GG_HIP(hipGetDeviceCount(&cnt));
+  if (cnt <= 0) return set_error(GG_ERR_HIP, "no HIP device visible");
+  if (device >= cnt) device = device % cnt;  // like general/evaluation.cu:184
+  GG_HIP(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  GG_HIP(hipGetDeviceProperties(&prop, device));
+  E.cus = prop.multiProcessorCount;
+  GG_HIP(hipStreamCreateWithFlags(&E.stream, hipStreamNonBlocking));
+  // Allow the large dynamic LDS allocation on every instance we may launch.
+  for (int bits : {8, 16, 32, 64})
+    for (int block : {256, 1024})
+      for (int vpt : {1, 2, 4, 8, 16, 32, 64})
+        for (int h : {0, 1}) {
+          TileKernel k = kernel_for({bits, block, vpt, h});
+          if (!k) continue;
+          GG_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     kLdsBytes));
+        }
+  E.device = device;
+  E.ready = true;
+  return GG_OK;
+}
+
+void finalize() {
+  if (!E.ready) return;
+  hipStreamSynchronize(E.stream);
+  hipStreamDestroy(E.stream);
+  E.stream = nullptr;
+  E.ready = false;
+}
+
+int max_bits_for(int n) {
+  for (int bits : {64, 32, 16, 8})
+    if (qcap_for(bits, n, kLdsBytes) >= 64) return bits;
+  return 0;
+}
+
+// Word width for one graph.  Single evaluations favour narrow words when the
+// graph has few source tiles (more workgroups in flight); batches favour the
+// widest word (fewest adjacency passes).
+int choose_bits(int n, int d, int b, int g, int c, int batch) {
+  int maxb = max_bits_for(n);
+  if (maxb == 0) return 0;
+  if (E.forced_bits) return std::min(E.forced_bits, maxb);
+  auto it = E.tune.find({n, d, b, g, batch > 1 ? 1 : 0});
+  if (it != E.tune.end() && it->second <= maxb) return it->second;
+  if (batch > 1) return maxb;
+  int bits = maxb;
+  const int srcs = b + c;
+  while (bits > 8 && (srcs + bits - 1) / bits < E.cus / 2) bits /= 2;
+  return bits;
+}
+
+// ---- Batch assembly -------------------------------------------------------
+
+struct Slot {  // one graph in an assembled launch set
+  int n, d, dp, b, g, c, bits, block, vpt, flags;
+  size_t adj_off, adjT_off, rel_off;  // element offsets in the device buffers
+  bool has_rel;
+};
+
+struct LaunchGroup {
+  KernelKey key;
+  int n_cap, qcap, unit_begin, unit_count;
+  size_t lds;
+};
+
+struct Assembly {
+  std::vector<Slot> slots;
+  std::vector<GraphDev> gdev;
+  std::vector<UnitDesc> units;
+  std::vector<GraphMeta> meta;
+  std::vector<LaunchGroup> groups;
+  size_t adj_elems = 0, adjT_elems = 0, rel_elems = 0;
+};
+
+int validate(const gg_graph &g) {
+  if (g.nodes < 2 || g.degree < 1 || g.based_nodes < 1 || g.groups < 1 || g.added_centers < 0)
+    return set_error(GG_ERR_INVALID, "invalid graph sizes (n=%d d=%d B=%d g=%d c=%d)", g.nodes,
+                     g.degree, g.based_nodes, g.groups, g.added_centers);
+  if ((int64_t)g.based_nodes * g.groups + g.added_centers != g.nodes)
+    return set_error(GG_ERR_INVALID, "based_nodes*groups+added_centers (%d*%d+%d) != nodes (%d)",
+                     g.based_nodes, g.groups, g.added_centers, g.nodes);
+  if (g.added_centers > g.based_nodes)
+    return set_error(GG_ERR_INVALID, "added_centers > based_nodes");
+  if (g.nodes > kMaxNodes)
+    return set_error(GG_ERR_TOO_LARGE, "nodes=%d above the device limit %d", g.nodes, kMaxNodes);
+  if (g.degree > 4096) return set_error(GG_ERR_INVALID, "degree %d too large", g.degree);
+  return GG_OK;
+}
+
+int assemble(int count, const gg_graph *gs, int hist_maxd, Assembly &A) {
+  A = Assembly();
+  A.slots.resize(count);
+  for (int i = 0; i < count; ++i) {
+    const gg_graph &g = gs[i];
+    int rc = validate(g);
+    if (rc) return rc;
+    Slot &s = A.slots[i];
+    s.n = g.nodes;
+    s.d = g.degree;
+    s.dp = (g.degree + 3) & ~3;
+    s.b = g.based_nodes;
+    s.g = g.groups;
+    s.c = g.added_centers;
+    s.flags = g.flags;
+    s.bits = choose_bits(s.n, s.d, s.b, s.g, s.c, count);
+    if (s.bits == 0) return set_error(GG_ERR_TOO_LARGE, "n=%d does not fit LDS", s.n);
+    block_vpt_for(s.n, &s.block, &s.vpt);
+    if (s.bits == 64 && s.vpt == 64) return set_error(GG_ERR_TOO_LARGE, "no 64-bit instance for n=%d", s.n);
+    s.adj_off = A.adj_elems;
+    A.adj_elems += (size_t)s.n * s.d;
+    s.adjT_off = A.adjT_elems;
+    A.adjT_elems += (size_t)s.n * s.dp;
+    s.has_rel = g.relabel != nullptr;
+    s.rel_off = A.rel_elems;
+    if (s.has_rel) A.rel_elems += s.n;
+  }
+  // Group graphs by kernel instance; inside a group, largest graphs first.
+  std::map<KernelKey, std::vector<int>> by_key;
+  for (int i = 0; i < count; ++i) {
+    const Slot &s = A.slots[i];
+    by_key[{s.bits, s.block, s.vpt, hist_maxd > 0}].push_back(i);
+  }
+  A.meta.resize(count);
+  A.gdev.resize(count);
+  for (auto &kv : by_key) {
+    std::vector<int> &ids = kv.second;
+    std::stable_sort(ids.begin(), ids.end(), [&](int x, int y) {
+      return (int64_t)A.slots[x].n * A.slots[x].d > (int64_t)A.slots[y].n * A.slots[y].d;
+    });
+    LaunchGroup lg;
+    lg.key = kv.first;
+    lg.n_cap = 0;
+    for (int i : ids) lg.n_cap = std::max(lg.n_cap, A.slots[i].n);
+    lg.n_cap = (lg.n_cap + 7) & ~7;
+    // Budget: when the reach array is small, cap LDS so several workgroups
+    // share a CU; large graphs take the whole 160 KiB.
+    size_t rbytes = (size_t)lg.n_cap * (kv.first.bits / 8);
+    size_t budget = rbytes * 2 + 4096 <= (size_t)kLdsBytes / 2 ? rbytes * 2 + 4096 : (size_t)kLdsBytes;
+    lg.qcap = qcap_for(kv.first.bits, lg.n_cap, budget);
+    if (lg.qcap < 64) return set_error(GG_ERR_TOO_LARGE, "LDS budget too small");
+    lg.lds = lds_bytes_for(kv.first.bits, lg.n_cap, lg.qcap);
+    lg.unit_begin = (int)A.units.size();
+    for (int i : ids) {
+      const Slot &s = A.slots[i];
+      GraphMeta &m = A.meta[i];
+      m.unit_begin = (int)A.units.size();
+      m.n = s.n;
+      m.groups = s.g;
+      const int bits = s.bits;
+      for (int f = 0; f < s.b; f += bits) A.units.push_back({i, f, std::min(bits, s.b - f), 0});
+      for (int f = 0; f < s.c; f += bits)
+        A.units.push_back({i, s.n - s.c + f, std::min(bits, s.c - f), 1});
+      m.unit_count = (int)A.units.size() - m.unit_begin;
+    }
+    lg.unit_count = (int)A.units.size() - lg.unit_begin;
+    A.groups.push_back(lg);
+  }
+  return GG_OK;
+}
+
+// Device buffers of one launch set (either the engine's single scratch or a plan's own).
+struct DevSet {
+  DevBuf adj_in, relabel_in, adjT, flags, gdesc, units, outs, meta, gout, hist;
+  PinBuf res_h;
+  void release() {
+    for (DevBuf *b : {&adj_in, &relabel_in, &adjT, &flags, &gdesc, &units, &outs, &meta, &gout, &hist})
+      b->release();
+    res_h.release();
+  }
+};
+
+int prepare_buffers(Assembly &A, DevSet &D, int hist_maxd, int32_t *hist_dev) {
+  const int count = (int)A.slots.size();
+  int rc;
+  if ((rc = D.adj_in.ensure(A.adj_elems * 4))) return rc;
+  if ((rc = D.relabel_in.ensure(std::max<size_t>(A.rel_elems, 1) * 4))) return rc;
+  if ((rc = D.adjT.ensure(A.adjT_elems * 2))) return rc;
+  if ((rc = D.flags.ensure((size_t)count * 2 * 4))) return rc;
+  if ((rc = D.gdesc.ensure((size_t)count * sizeof(GraphDev)))) return rc;
+  if ((rc = D.units.ensure(A.units.size() * sizeof(UnitDesc)))) return rc;
+  if ((rc = D.outs.ensure(A.units.size() * sizeof(UnitOut)))) return rc;
+  if ((rc = D.meta.ensure((size_t)count * sizeof(GraphMeta)))) return rc;
+  if ((rc = D.gout.ensure((size_t)count * sizeof(GraphOut)))) return rc;
+  if ((rc = D.res_h.ensure((size_t)count * sizeof(GraphOut)))) return rc;
+  for (int i = 0; i < count; ++i) {
+    const Slot &s = A.slots[i];
+    GraphDev &g = A.gdev[i];
+    g.adjT = (const uint16_t *)D.adjT.p + s.adjT_off;
+    g.flags = (int32_t *)D.flags.p + 2 * i;
+    g.n = s.n;
+    g.dp = s.dp;
+    g.split = s.b * s.g;
+    g.c = s.c;
+    g.push_ok = 1;
+    g.groups = s.g;
+    g.hist_maxd = hist_maxd;
+    g.pad = 0;
+    g.hist = hist_dev;
+  }
+  GG_HIP(hipMemcpyAsync(D.gdesc.p, A.gdev.data(), count * sizeof(GraphDev), hipMemcpyHostToDevice, E.stream));
+  GG_HIP(hipMemcpyAsync(D.units.p, A.units.data(), A.units.size() * sizeof(UnitDesc),
+                        hipMemcpyHostToDevice, E.stream));
+  GG_HIP(hipMemcpyAsync(D.meta.p, A.meta.data(), count * sizeof(GraphMeta), hipMemcpyHostToDevice,
+                        E.stream));
+  return GG_OK;
+}
+
+// Enqueue prep + tile BFS + combine for an assembled set whose adjacency
+// already sits in `adj_src[i]` (device pointers) -- graph-capturable (no
+// memset nodes, no host sync).
+int enqueue(Assembly &A, DevSet &D, const int32_t *const *adj_src, const int32_t *const *rel_src,
+            const int *check_sym, hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr) {
+  const int count = (int)A.slots.size();
+  // flags cleared by a kernel-free path: hipMemsetAsync is avoided so the
+  // sequence can be captured (a small fill is folded into the prep launch
+  // ordering below through a dedicated clear kernel).
+  hipLaunchKernelGGL(gg_clear_i32_kernel, dim3((2 * count + 255) / 256), dim3(256), 0, E.stream,
+                     (int32_t *)D.flags.p, 2 * count);
+  for (int i = 0; i < count; ++i) {
+    const Slot &s = A.slots[i];
+    hipLaunchKernelGGL(gg_prep_kernel, dim3((s.n + 255) / 256), dim3(256), 0, E.stream,
+                       adj_src[i], rel_src[i], s.n, s.d, s.dp,
+                       (uint16_t *)D.adjT.p + s.adjT_off, (int32_t *)D.flags.p + 2 * i,
+                       check_sym[i]);
+  }
+  if (ev_begin) GG_HIP(hipEventRecord(ev_begin, E.stream));
+  for (const LaunchGroup &lg : A.groups) {
+    TileKernel k = kernel_for(lg.key);
+    if (!k) return set_error(GG_ERR_STATE, "no kernel instance bits=%d block=%d vpt=%d", lg.key.bits,
+                             lg.key.block, lg.key.vpt);
+    hipLaunchKernelGGL(k, dim3(lg.unit_count), dim3(lg.key.block), lg.lds, E.stream,
+                       (const GraphDev *)D.gdesc.p, (const UnitDesc *)D.units.p, (UnitOut *)D.outs.p,
+                       lg.unit_begin, lg.n_cap, lg.qcap);
+  }
+  if (ev_end) GG_HIP(hipEventRecord(ev_end, E.stream));
+  hipLaunchKernelGGL(gg_combine_kernel, dim3((count + 127) / 128), dim3(128), 0, E.stream,
+                     (const GraphMeta *)D.meta.p, (const UnitDesc *)D.units.p,
+                     (const UnitOut *)D.outs.p, (GraphOut *)D.gout.p, count);
+  GG_HIP(hipGetLastError());
+  GG_HIP(hipMemcpyAsync(D.res_h.p, D.gout.p, count * sizeof(GraphOut), hipMemcpyDeviceToHost, E.stream));
+  return GG_OK;
+}
+
+int collect(Assembly &A, DevSet &D, gg_result *out) {
+  GG_HIP(hipStreamSynchronize(E.stream));
+  const GraphOut *r = (const GraphOut *)D.res_h.p;
+  int rc = GG_OK;
+  for (size_t i = 0; i < A.slots.size(); ++i) {
+    out[i].connected = r[i].connected;
+    out[i].diameter = r[i].diameter;
+    out[i].max_ecc = r[i].max_ecc;
+    out[i].word_bits = A.slots[i].bits;
+    out[i].sum = r[i].sum;
+    if (r[i].error)
+      rc = set_error(GG_ERR_INVALID, "graph %zu: adjacency entry outside [0,%d)", i, A.slots[i].n);
+  }
+  return rc;
+}
+
+static DevSet g_single;
+
+// A host relabel table must be a permutation of 0..n-1: every row of the
+// transposed table is then written exactly once.
+int check_permutation(const int32_t *p, int n) {
+  std::vector<unsigned char> seen(n, 0);
+  for (int i = 0; i < n; ++i) {
+    if (p[i] < 0 || p[i] >= n || seen[p[i]])
+      return set_error(GG_ERR_INVALID, "relabel is not a permutation (entry %d = %d)", i, p[i]);
+    seen[p[i]] = 1;
+  }
+  return GG_OK;
+}
+
+int eval_host(int count, const gg_graph *gs, gg_result *out, bool device_ptrs, int hist_maxd,
+              int32_t *hist_host) {
+  // Argument checks first: they need no device.
+  int rc;
+  for (int i = 0; i < count; ++i) {
+    if ((rc = validate(gs[i]))) return rc;
+    if (!device_ptrs && gs[i].relabel && (rc = check_permutation(gs[i].relabel, gs[i].nodes)))
+      return rc;
+  }
+  if ((rc = init(-1))) return rc;
+  Assembly A;
+  if ((rc = assemble(count, gs, hist_maxd, A))) return rc;
+  DevSet &D = g_single;
+  int32_t *hist_dev = nullptr;
+  size_t hist_bytes = 0;
+  if (hist_maxd > 0) {
+    hist_bytes = (size_t)gs[0].nodes * (hist_maxd + 1) * 4;
+    if ((rc = D.hist.ensure(hist_bytes))) return rc;
+    hist_dev = (int32_t *)D.hist.p;
+    GG_HIP(hipMemsetAsync(hist_dev, 0, hist_bytes, E.stream));
+  }
+  if ((rc = prepare_buffers(A, D, hist_maxd, hist_dev))) return rc;
+  std::vector<const int32_t *> src(count), rel(count);
+  std::vector<int> sym(count);
+  for (int i = 0; i < count; ++i) {
+    const Slot &s = A.slots[i];
+    sym[i] = (gs[i].flags & GG_GRAPH_SYMMETRIC) ? 0 : 1;
+    if (device_ptrs) {
+      src[i] = gs[i].adj;
+      rel[i] = gs[i].relabel;
+    } else {
+      int32_t *dst = (int32_t *)D.adj_in.p + s.adj_off;
+      GG_HIP(hipMemcpyAsync(dst, gs[i].adj, (size_t)s.n * s.d * 4, hipMemcpyHostToDevice, E.stream));
+      src[i] = dst;
+      rel[i] = nullptr;
+      if (s.has_rel) {
+        int32_t *rd = (int32_t *)D.relabel_in.p + s.rel_off;
+        GG_HIP(hipMemcpyAsync(rd, gs[i].relabel, (size_t)s.n * 4, hipMemcpyHostToDevice, E.stream));
+        rel[i] = rd;
+      }
+    }
+  }
+  if ((rc = enqueue(A, D, src.data(), rel.data(), sym.data()))) return rc;
+  if (hist_maxd > 0)
+    GG_HIP(hipMemcpyAsync(hist_host, hist_dev, hist_bytes, hipMemcpyDeviceToHost, E.stream));
+  return collect(A, D, out);
+}
+
+// ---- Plans ---------------------------------------------------------------
+
+struct Plan {
+  Assembly A;
+  DevSet D;
+  std::vector<const int32_t *> src, rel;
+  std::vector<int> sym;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t ev_begin = nullptr, ev_end = nullptr;  // around the tile-BFS launches
+  bool launched = false;
+};
+
+}  // namespace gg
+
+using namespace gg;
+
+struct gg_plan : Plan {};
+
+extern "C" {
+
+int gg_init(int device) { return gg::init(device); }
+void gg_finalize(void) {
+  g_single.release();
+  gg::finalize();
+}
+const char *gg_last_error(void) { return gg::last_error(); }
+int gg_device_count(void) {
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+  return c;
+}
+
+int gg_apsp(const gg_graph *g, gg_result *out) {
+  if (!g || !out || !g->adj) return set_error(GG_ERR_INVALID, "null argument");
+  return eval_host(1, g, out, false, 0, nullptr);
+}
+
+int gg_apsp_device(const gg_graph *g, gg_result *out) {
+  if (!g || !out || !g->adj) return set_error(GG_ERR_INVALID, "null argument");
+  return eval_host(1, g, out, true, 0, nullptr);
+}
+
+int gg_eval_batch(int count, const gg_graph *graphs, gg_result *out) {
+  if (count <= 0 || !graphs || !out) return set_error(GG_ERR_INVALID, "bad batch arguments");
+  for (int i = 0; i < count; ++i)
+    if (!graphs[i].adj) return set_error(GG_ERR_INVALID, "graph %d has no adjacency", i);
+  return eval_host(count, graphs, out, false, 0, nullptr);
+}
+
+int gg_distance_histogram(int nodes, int degree, const int32_t *adj, int maxd, int32_t *hist) {
+  if (!adj || !hist || maxd < 1) return set_error(GG_ERR_INVALID, "bad histogram arguments");
+  gg_graph g = {nodes, degree, nodes, 1, 0, 0, adj, nullptr};
+  gg_result r;
+  return eval_host(1, &g, &r, false, maxd, hist);
+}
+
+int gg_set_word_bits(int bits) {
+  if (bits != 0 && bits != 8 && bits != 16 && bits != 32 && bits != 64)
+    return set_error(GG_ERR_INVALID, "word bits must be 0/8/16/32/64");
+  engine().forced_bits = bits;
+  return GG_OK;
+}
+
+// Times every feasible word width on a seeded graph of the given shape
+// (batch copies in one plan) and records the fastest in the tuning table.
+int gg_autotune(int nodes, int degree, int based_nodes, int groups, int batch, int reps,
+                int *best_bits, double *best_ms) {
+  if (nodes < 2 || degree < 1 || batch < 1 || reps < 1 || groups < 1 || based_nodes < 1)
+    return set_error(GG_ERR_INVALID, "bad autotune arguments");
+  int rc = gg::init(-1);
+  if (rc) return rc;
+  const int c = nodes - based_nodes * groups;
+  std::vector<int32_t> edges((size_t)nodes * degree);
+  rc = groups > 1 && c == 0 ? gg_random_symmetric(nodes, degree, groups, 12345, edges.data())
+                            : gg_random_regular(nodes, degree, 12345, edges.data());
+  if (rc) return rc;
+  std::vector<int32_t> adj((size_t)nodes * degree);
+  if ((rc = gg_edges_to_adj(nodes, nodes * degree / 2, degree, edges.data(), adj.data()))) return rc;
+  gg_graph g = {nodes, degree, based_nodes, groups, c, GG_GRAPH_SYMMETRIC, adj.data(), nullptr};
+  std::vector<gg_graph> shapes(batch, g);
+  std::vector<gg_result> res(batch);
+  const int saved = engine().forced_bits;
+  int best = 0;
+  double best_t = 1e30;
+  for (int bits : {8, 16, 32, 64}) {
+    if (bits > max_bits_for(nodes)) continue;
+    engine().forced_bits = bits;
+    gg_plan *p = gg_plan_create(batch, shapes.data());
+    if (!p) {
+      engine().forced_bits = saved;
+      return GG_ERR_HIP;
+    }
+    for (int i = 0; i < batch; ++i) gg_plan_set_adj(p, i, adj.data());
+    rc = gg_plan_run(p, res.data());  // warm-up
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    hipEventRecord(e0, engine().stream);
+    for (int r = 0; r < reps && !rc; ++r) rc = gg_plan_launch(p);
+    hipEventRecord(e1, engine().stream);
+    if (!rc) rc = gg_plan_wait(p, res.data());
+    float ms = 0;
+    hipEventSynchronize(e1);
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    gg_plan_destroy(p);
+    if (rc) {
+      engine().forced_bits = saved;
+      return rc;
+    }
+    double t = ms / reps;
+    if (t < best_t) {
+      best_t = t;
+      best = bits;
+    }
+  }
+  engine().forced_bits = saved;
+  engine().tune[{nodes, degree, based_nodes, groups, batch > 1 ? 1 : 0}] = best;
+  if (best_bits) *best_bits = best;
+  if (best_ms) *best_ms = best_t;
+  return GG_OK;
+}
+
+int gg_tune_table_save(const char *path) {
+  FILE *f = fopen(path, "w");
+  if (!f) return set_error(GG_ERR_INVALID, "cannot write %s", path);
+  fprintf(f, "# nodes degree based_nodes groups batched word_bits\n");
+  for (auto &kv : engine().tune)
+    fprintf(f, "%d %d %d %d %d %d\n", kv.first.n, kv.first.d, kv.first.b, kv.first.g,
+            kv.first.batch, kv.second);
+  fclose(f);
+  return GG_OK;
+}
+
+int gg_tune_table_load(const char *path) {
+  FILE *f = fopen(path, "r");
+  if (!f) return set_error(GG_ERR_INVALID, "cannot read %s", path);
+  char line[256];
+  int loaded = 0;
+  while (fgets(line, sizeof line, f)) {
+    if (line[0] == '#') continue;
+    int n, d, b, g, bt, bits;
+    if (sscanf(line, "%d %d %d %d %d %d", &n, &d, &b, &g, &bt, &bits) == 6 &&
+        (bits == 8 || bits == 16 || bits == 32 || bits == 64)) {
+      engine().tune[{n, d, b, g, bt}] = bits;
+      ++loaded;
+    }
+  }
+  fclose(f);
+  return loaded;
+}
+
+gg_plan *gg_plan_create(int count, const gg_graph *shapes) {
+  if (gg::init(-1)) return nullptr;
+  if (count <= 0 || !shapes) {
+    set_error(GG_ERR_INVALID, "bad plan arguments");
+    return nullptr;
+  }
+  for (int i = 0; i < count; ++i)
+    if (shapes[i].relabel && check_permutation(shapes[i].relabel, shapes[i].nodes)) return nullptr;
+  gg_plan *p = new gg_plan();
+  if (assemble(count, shapes, 0, p->A) || prepare_buffers(p->A, p->D, 0, nullptr)) {
+    p->D.release();
+    delete p;
+    return nullptr;
+  }
+  p->src.resize(count);
+  p->rel.assign(count, nullptr);
+  p->sym.resize(count);
+  for (int i = 0; i < count; ++i) {
+    p->src[i] = (const int32_t *)p->D.adj_in.p + p->A.slots[i].adj_off;
+    p->sym[i] = (shapes[i].flags & GG_GRAPH_SYMMETRIC) ? 0 : 1;
+    if (shapes[i].relabel) {
+      int32_t *rd = (int32_t *)p->D.relabel_in.p + p->A.slots[i].rel_off;
+      if (hipMemcpyAsync(rd, shapes[i].relabel, (size_t)shapes[i].nodes * 4, hipMemcpyHostToDevice,
+                         engine().stream) != hipSuccess) {
+        set_error(GG_ERR_HIP, "relabel upload failed");
+        p->D.release();
+        delete p;
+        return nullptr;
+      }
+      p->rel[i] = rd;
+    }
+  }
+  // Capture prep + BFS + combine + result copy once; replays reuse it.
+  hipStream_t s = engine().stream;
+  if (hipStreamSynchronize(s) != hipSuccess ||
+      hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    set_error(GG_ERR_HIP, "stream capture could not start");
+    p->D.release();
+    delete p;
+    return nullptr;
+  }
+  if (hipEventCreate(&p->ev_begin) != hipSuccess || hipEventCreate(&p->ev_end) != hipSuccess) {
+    set_error(GG_ERR_HIP, "event creation failed");
+    p->D.release();
+    delete p;
+    return nullptr;
+  }
+  int rc = enqueue(p->A, p->D, p->src.data(), p->rel.data(), p->sym.data(), p->ev_begin, p->ev_end);
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamEndCapture(s, &graph);
+  if (rc || e != hipSuccess || !graph ||
+      hipGraphInstantiate(&p->exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+    if (!rc) set_error(GG_ERR_HIP, "graph capture/instantiate failed: %s", hipGetErrorString(e));
+    if (graph) hipGraphDestroy(graph);
+    p->D.release();
+    delete p;
+    return nullptr;
+  }
+  p->graph = graph;
+  return p;
+}
+
+int gg_plan_set_adj(gg_plan *p, int index, const int32_t *adj_host) {
+  if (!p || index < 0 || index >= (int)p->A.slots.size() || !adj_host)
+    return set_error(GG_ERR_INVALID, "bad plan slot");
+  const Slot &s = p->A.slots[index];
+  GG_HIP(hipMemcpyAsync((int32_t *)p->D.adj_in.p + s.adj_off, adj_host, (size_t)s.n * s.d * 4,
+                        hipMemcpyHostToDevice, engine().stream));
+  return GG_OK;
+}
+
+int32_t *gg_plan_adj_device(gg_plan *p, int index) {
+  if (!p || index < 0 || index >= (int)p->A.slots.size()) return nullptr;
+  return (int32_t *)p->D.adj_in.p + p->A.slots[index].adj_off;
+}
+
+int gg_plan_launch(gg_plan *p) {
+  if (!p || !p->exec) return set_error(GG_ERR_STATE, "plan not ready");
+  GG_HIP(hipGraphLaunch(p->exec, engine().stream));
+  p->launched = true;
+  return GG_OK;
+}
+
+int gg_plan_wait(gg_plan *p, gg_result *out) {
+  if (!p || !p->launched) return set_error(GG_ERR_STATE, "plan not launched");
+  p->launched = false;
+  return collect(p->A, p->D, out);
+}
+
+int gg_plan_run(gg_plan *p, gg_result *out) {
+  int rc = gg_plan_launch(p);
+  if (rc) return rc;
+  return gg_plan_wait(p, out);
+}
+
+float gg_plan_kernel_ms(gg_plan *p) {
+  if (!p || !p->ev_begin) return -1.0f;
+  float ms = -1.0f;
+  if (hipEventElapsedTime(&ms, p->ev_begin, p->ev_end) != hipSuccess) return -1.0f;
+  return ms;
+}
+
+int gg_synchronize(void) {
+  if (!engine().ready) return GG_OK;
+  GG_HIP(hipStreamSynchronize(engine().stream));
+  GG_HIP(hipDeviceSynchronize());
+  return GG_OK;
+}
+
+void gg_plan_destroy(gg_plan *p) {
+  if (!p) return;
+  hipStreamSynchronize(engine().stream);
+  if (p->exec) hipGraphExecDestroy(p->exec);
+  if (p->graph) hipGraphDestroy(p->graph);
+  if (p->ev_begin) hipEventDestroy(p->ev_begin);
+  if (p->ev_end) hipEventDestroy(p->ev_end);
+  p->D.release();
+  delete p;
+}
+
+}  // extern "C"

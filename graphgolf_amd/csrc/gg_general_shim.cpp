@@ -1,0 +1,50 @@
+// gg_general_shim.cpp -- libgg_general.so: the reference's general
+// evaluation() ABI (general/common.h:112-113) over libggeval.so.
+#include <cfloat>
+#include <climits>
+#include <cstdio>
+#include <cstdlib>
+
+#include "gg_eval.h"
+#include "gg_general.h"
+
+// The reference links its own timer.c (general/timer.c:23-38); when it is
+// present those strong definitions win over these no-op defaults.
+extern "C" __attribute__((weak)) void timer_start(const int) {}
+extern "C" __attribute__((weak)) void timer_stop(const int) {}
+static const int kTimerApsp = 2;  // TIMER_APSP, general/common.h:41
+
+// Fatal errors end the process, as the reference's ERROR() does
+// (general/common.h:67-70).
+static void fatal(const char *what) {
+  fprintf(stderr, "gg evaluation: %s: %s\n", what, gg_last_error());
+  fflush(stderr);
+  exit(1);
+}
+
+extern "C" void init_matrix_dev(const int, const int, const int, const int) {
+  if (gg_init(-1) != GG_OK) fatal("init_matrix_dev");
+}
+
+extern "C" void finalize_matrix_dev(void) { gg_finalize(); }
+
+extern "C" bool evaluation(const int nodes, const int based_nodes, const int groups, const int lines,
+                           const int degree, int *adj, int *diameter, double *ASPL,
+                           const int added_centers, const int algo) {
+  (void)lines;
+  (void)algo;  // BFS / MATRIX_OP / MEM_SAVING: one device path (see gg_general.h)
+  timer_start(kTimerApsp);
+  gg_graph g = {nodes, degree, based_nodes, groups, added_centers, 0, adj, nullptr};
+  gg_result r;
+  if (gg_apsp(&g, &r) != GG_OK) fatal("evaluation");
+  bool ok = r.connected && r.diameter < nodes;  // general/evaluation.c:254
+  if (ok) {
+    *diameter = r.diameter;
+    *ASPL = (double)r.sum / (((double)nodes - 1) * nodes);
+  } else {
+    *diameter = INT_MAX;
+    *ASPL = DBL_MAX;
+  }
+  timer_stop(kTimerApsp);
+  return ok;
+}

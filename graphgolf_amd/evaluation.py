@@ -1,0 +1,229 @@
+"""Host-side mirror of the reference's evaluator interface over the gfx950
+kernels in libggeval.so.
+
+``evaluation`` / ``grid_evaluation`` keep the reference's argument meaning
+and return convention (general/evaluation.c:375-391, grid/evaluation.c:192-209);
+``apsp`` / ``eval_batch`` / ``Plan`` expose the raw integers (diameter, S).
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _lib as L
+
+INT_MAX = 2**31 - 1
+DBL_MAX = 1.7976931348623157e308
+BFS, MATRIX_OP, MATRIX_OP_MEM_SAVING = 0, 1, 2  # general/common.h:58-60
+
+
+@dataclass
+class Result:
+    connected: bool
+    diameter: int      # raw level count; nodes+1 when disconnected
+    max_ecc: int
+    sum: int           # S, the ordered-pair distance sum (symmetry weighted)
+    word_bits: int
+    nodes: int
+
+    @property
+    def aspl(self) -> float:
+        n = self.nodes
+        return self.sum / ((float(n) - 1) * n)
+
+
+def init(device: int = -1) -> None:
+    L.check(L.lib().gg_init(device))
+
+
+def _graph(adj: np.ndarray, based_nodes, groups, added_centers, relabel, symmetric=False):
+    n, d = adj.shape
+    g = L.GGGraph()
+    g.nodes, g.degree = n, d
+    g.groups = groups
+    g.added_centers = added_centers
+    g.based_nodes = (n - added_centers) // groups if based_nodes is None else based_nodes
+    g.flags = L.GG_GRAPH_SYMMETRIC if symmetric else 0
+    g.adj = L.ptr(adj)
+    g.relabel = L.ptr(relabel) if relabel is not None else ctypes.cast(None, L.i32p)
+    return g
+
+
+def _result(r: L.GGResult, n: int) -> Result:
+    return Result(bool(r.connected), int(r.diameter), int(r.max_ecc), int(r.sum),
+                  int(r.word_bits), n)
+
+
+def apsp(adj, based_nodes: Optional[int] = None, groups: int = 1, added_centers: int = 0,
+         relabel=None, symmetric: bool = False) -> Result:
+    """One evaluation on the GPU (adjacency from host memory)."""
+    adj = L.c32(adj)
+    rel = L.c32(relabel) if relabel is not None else None
+    g = _graph(adj, based_nodes, groups, added_centers, rel, symmetric)
+    r = L.GGResult()
+    L.check(L.lib().gg_apsp(ctypes.byref(g), ctypes.byref(r)))
+    return _result(r, adj.shape[0])
+
+
+def apsp_device(adj_dev_ptr: int, nodes: int, degree: int, based_nodes=None, groups=1,
+                added_centers=0, symmetric=False) -> Result:
+    """One evaluation whose adjacency already sits in device memory (an
+    int32 [nodes][degree] buffer at ``adj_dev_ptr``, e.g. a torch tensor's
+    data_ptr())."""
+    g = L.GGGraph()
+    g.nodes, g.degree, g.groups, g.added_centers = nodes, degree, groups, added_centers
+    g.based_nodes = (nodes - added_centers) // groups if based_nodes is None else based_nodes
+    g.flags = L.GG_GRAPH_SYMMETRIC if symmetric else 0
+    g.adj = ctypes.cast(ctypes.c_void_p(adj_dev_ptr), L.i32p)
+    g.relabel = ctypes.cast(None, L.i32p)
+    r = L.GGResult()
+    L.check(L.lib().gg_apsp_device(ctypes.byref(g), ctypes.byref(r)))
+    return _result(r, nodes)
+
+
+def eval_batch(graphs: Sequence[dict]) -> list:
+    """Score many graphs in one launch set.  Each item: dict(adj=..., and
+    optionally based_nodes, groups, added_centers, relabel, symmetric)."""
+    keep = []
+    arr = (L.GGGraph * len(graphs))()
+    for i, item in enumerate(graphs):
+        adj = L.c32(item["adj"])
+        rel = item.get("relabel")
+        rel = L.c32(rel) if rel is not None else None
+        keep.append((adj, rel))
+        arr[i] = _graph(adj, item.get("based_nodes"), item.get("groups", 1),
+                        item.get("added_centers", 0), rel, item.get("symmetric", False))
+    out = (L.GGResult * len(graphs))()
+    L.check(L.lib().gg_eval_batch(len(graphs), arr, out))
+    return [_result(out[i], keep[i][0].shape[0]) for i in range(len(graphs))]
+
+
+class Plan:
+    """A batch of fixed graph shapes with device buffers and a captured
+    hipGraph; ``run`` uploads nothing, ``set_adj`` uploads one slot."""
+
+    def __init__(self, shapes: Sequence[dict]):
+        self.n = len(shapes)
+        arr = (L.GGGraph * self.n)()
+        self._keep = []
+        self.nodes = []
+        for i, s in enumerate(shapes):
+            nodes, degree = s["nodes"], s["degree"]
+            dummy = np.zeros((nodes, degree), dtype=np.int32)
+            rel = s.get("relabel")
+            rel = L.c32(rel) if rel is not None else None
+            self._keep.append((dummy, rel))
+            arr[i] = _graph(dummy, s.get("based_nodes"), s.get("groups", 1),
+                            s.get("added_centers", 0), rel, s.get("symmetric", False))
+            self.nodes.append(nodes)
+        p = L.lib().gg_plan_create(self.n, arr)
+        if not p:
+            raise L.GGError(-4, (L.lib().gg_last_error() or b"").decode())
+        self._p = ctypes.c_void_p(p)
+        self._out = (L.GGResult * self.n)()
+
+    def set_adj(self, index: int, adj) -> None:
+        a = L.c32(adj)
+        L.check(L.lib().gg_plan_set_adj(self._p, index, L.ptr(a)))
+        self._keep.append((a, None))  # keep alive until the async copy is done
+
+    def device_adj(self, index: int) -> int:
+        return int(L.lib().gg_plan_adj_device(self._p, index))
+
+    def run(self) -> list:
+        L.check(L.lib().gg_plan_run(self._p, self._out))
+        self._keep = self._keep[: self.n]
+        return [_result(self._out[i], self.nodes[i]) for i in range(self.n)]
+
+    def launch(self) -> None:
+        L.check(L.lib().gg_plan_launch(self._p))
+
+    def wait(self) -> list:
+        L.check(L.lib().gg_plan_wait(self._p, self._out))
+        self._keep = self._keep[: self.n]
+        return [_result(self._out[i], self.nodes[i]) for i in range(self.n)]
+
+    def kernel_ms(self) -> float:
+        """Tile-BFS kernel time of the last completed replay (HIP events)."""
+        return float(L.lib().gg_plan_kernel_ms(self._p))
+
+    def close(self) -> None:
+        if getattr(self, "_p", None):
+            L.lib().gg_plan_destroy(self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def synchronize() -> None:
+    L.check(L.lib().gg_synchronize())
+
+
+def autotune(nodes: int, degree: int, based_nodes=None, groups: int = 1, batch: int = 1,
+             reps: int = 5):
+    """Time every feasible word width for this shape; the fastest is used
+    from then on.  Returns (bits, ms per replay)."""
+    bits = ctypes.c_int(0)
+    ms = ctypes.c_double(0)
+    b = nodes // groups if based_nodes is None else based_nodes
+    L.check(L.lib().gg_autotune(nodes, degree, b, groups, batch, reps, ctypes.byref(bits),
+                                ctypes.byref(ms)))
+    return bits.value, ms.value
+
+
+def tune_table_save(path: str) -> None:
+    L.check(L.lib().gg_tune_table_save(path.encode()))
+
+
+def tune_table_load(path: str) -> int:
+    n = L.lib().gg_tune_table_load(path.encode())
+    if n < 0:
+        L.check(n)
+    return n
+
+
+def set_word_bits(bits: int) -> None:
+    """Force the LDS word width (8/16/32/64 sources per word); 0 = automatic."""
+    L.check(L.lib().gg_set_word_bits(bits))
+
+
+def distance_histogram(adj, maxd: int) -> np.ndarray:
+    """hist[v, k] = vertices at distance exactly k from v (k < maxd);
+    hist[v, maxd] = vertices at distance >= maxd."""
+    adj = L.c32(adj)
+    n, d = adj.shape
+    h = np.zeros((n, maxd + 1), dtype=np.int32)
+    L.check(L.lib().gg_distance_histogram(n, d, L.ptr(adj), maxd, L.ptr(h)))
+    return h
+
+
+# ---- The reference's entry points, same argument meaning -------------------
+
+def evaluation(nodes: int, based_nodes: int, groups: int, lines: int, degree: int, adj,
+               added_centers: int = 0, algo: int = MATRIX_OP):
+    """general/evaluation.c:375-391 -> (connected, diameter, ASPL); a
+    disconnected graph gives (False, INT_MAX, DBL_MAX)."""
+    a = L.c32(adj).reshape(nodes, degree)
+    r = apsp(a, based_nodes, groups, added_centers)
+    if r.connected and r.diameter < nodes:
+        return True, r.diameter, r.sum / ((float(nodes) - 1) * nodes)
+    return False, INT_MAX, DBL_MAX
+
+
+def grid_evaluation(nodes: int, degree: int, groups: int, adjacency, based_nodes: int,
+                    height: int, based_height: int, enable_bfs: bool, rotate_hash):
+    """grid/evaluation.c:192-209 -> (connected, diameter, ASPL or None).
+    Disconnected: diameter = nodes+1 (matrix) or the largest eccentricity
+    (enable_bfs); ASPL is not produced."""
+    a = L.c32(adjacency).reshape(nodes, degree)
+    r = apsp(a, based_nodes, groups, 0, relabel=rotate_hash)
+    if r.connected and not (r.diameter > nodes):
+        return True, r.diameter, r.sum / ((float(nodes) - 1) * nodes)
+    return False, (r.max_ecc if enable_bfs else nodes + 1), None

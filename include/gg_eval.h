@@ -1,0 +1,125 @@
+/*
+ * gg_eval.h -- C ABI of the MI355X (gfx950) APSP evaluator, libggeval.so.
+ *
+ * The evaluator computes, for a degree-regular graph given as an int32
+ * adjacency table adj[nodes][degree], the quantities the reference's
+ * evaluation() derives from its bit-parallel BFS (general/evaluation.c:153-263,
+ * grid/evaluation.c:122-190):
+ *   - the diameter (the level at which every source tile reached every vertex),
+ *   - S, the integer sum of ordered-pair distances with the reference's
+ *     symmetry weights (groups, added centres),
+ * bit-exactly.  ASPL is S / (((double)nodes - 1) * nodes), formed by the caller
+ * exactly as the reference does.
+ *
+ * Plain pointers and sizes only; no torch types.  Thread model: one host
+ * thread at a time per process (like the reference's evaluation()).
+ *
+ * The reference-compatible entry points, with the reference's exact
+ * signatures, live in gg_general.h and gg_grid.h (separate libraries,
+ * because both are named evaluation()).
+ */
+#ifndef GG_EVAL_H
+#define GG_EVAL_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Return codes. */
+#define GG_OK 0
+#define GG_ERR_INVALID (-1)   /* bad argument (sizes, ranges, adjacency entry out of range) */
+#define GG_ERR_TOO_LARGE (-2) /* nodes beyond what the device kernels support */
+#define GG_ERR_HIP (-3)       /* a HIP runtime call failed; see gg_last_error() */
+#define GG_ERR_STATE (-4)     /* library not initialised / plan misuse */
+
+/* Graph flags. */
+#define GG_GRAPH_SYMMETRIC 1  /* caller guarantees adj is symmetric (u in adj[v] <=> v in adj[u],
+                                 with multiplicity); skips the device-side check */
+
+/* One graph to score.  `adj` is row-major int32 [nodes][degree].  `relabel`
+ * (may be NULL) is a permutation of 0..nodes-1 applied to every vertex id
+ * before the BFS, as grid/evaluation.c:148-153 does with rotate_hash; the
+ * base sources are then the vertices relabelled to 0..based_nodes-1. */
+typedef struct gg_graph {
+  int32_t nodes;
+  int32_t degree;
+  int32_t based_nodes;   /* B: sources of the base pass are 0..B-1 */
+  int32_t groups;        /* g: rotational symmetry factor (1 = none) */
+  int32_t added_centers; /* c: centre vertices nodes-c..nodes-1 (general/evaluation.c:205) */
+  int32_t flags;         /* GG_GRAPH_* */
+  const int32_t *adj;
+  const int32_t *relabel;
+} gg_graph;
+
+/* Result of one evaluation. */
+typedef struct gg_result {
+  int32_t connected;  /* 1 when every source tile reached every vertex */
+  int32_t diameter;   /* raw level count (max over tiles of kk+1, at least 1); nodes+1 when
+                         disconnected, as the reference's loop leaves it */
+  int32_t max_ecc;    /* largest level at which any bit was added (the eccentricity the
+                         reference's grid bfs() reports, grid/evaluation.c:95) */
+  int32_t word_bits;  /* sources per tile word used (8/16/32/64) */
+  uint64_t sum;       /* S = sum over ordered pairs of distances, symmetry weighted
+                         (general/evaluation.c:199-200,241-249); meaningful when connected */
+} gg_result;
+
+/* Lifecycle.  gg_init(device) selects the HIP device (-1: $GG_DEVICE or 0) and
+ * creates the stream; it is idempotent.  The reference's counterpart is
+ * init_matrix_dev()/finalize_matrix_dev() (general/evaluation.cu:178-203). */
+int gg_init(int device);
+void gg_finalize(void);
+const char *gg_last_error(void);
+int gg_device_count(void);
+
+/* One evaluation, adjacency in host memory (copied to the device each call,
+ * like general/evaluation.cu:83).  Replaces the body of evaluation()
+ * (general/evaluation.c:375-391). */
+int gg_apsp(const gg_graph *g, gg_result *out);
+
+/* One evaluation whose adjacency (and relabel) pointers are DEVICE pointers
+ * (already resident in HBM).  Same contract as gg_apsp. */
+int gg_apsp_device(const gg_graph *g, gg_result *out);
+
+/* Score `count` graphs of any sizes in one set of launches (host adjacency).
+ * out[i] corresponds to graphs[i].  No reference counterpart: the batched
+ * scorer for simulated-annealing proposals. */
+int gg_eval_batch(int count, const gg_graph *graphs, gg_result *out);
+
+/* ---- Batched plans: device buffers + a captured hipGraph, replayed per batch.
+ * A plan is created for a fixed list of graph shapes; the caller writes each
+ * slot's adjacency (host upload, or device pointer for device-side producers)
+ * and replays. */
+typedef struct gg_plan gg_plan;
+gg_plan *gg_plan_create(int count, const gg_graph *shapes);
+int gg_plan_set_adj(gg_plan *p, int index, const int32_t *adj_host);
+int32_t *gg_plan_adj_device(gg_plan *p, int index); /* device buffer of slot `index` */
+int gg_plan_run(gg_plan *p, gg_result *out);        /* replay + wait + results */
+int gg_plan_launch(gg_plan *p);                     /* replay only (async) */
+int gg_plan_wait(gg_plan *p, gg_result *out);       /* wait for the last launch */
+void gg_plan_destroy(gg_plan *p);
+/* Duration (ms, HIP events on the evaluator's stream) of the tile-BFS
+ * kernels in the plan's last completed replay; -1 if unavailable. */
+float gg_plan_kernel_ms(gg_plan *p);
+/* Waits for all work of the evaluator's stream and the device. */
+int gg_synchronize(void);
+
+/* ---- Tuning.  Word width (8/16/32/64 sources per LDS word) and LDS budget
+ * per graph shape.  gg_set_word_bits(0) restores automatic choice (autotuned
+ * table when loaded, else the built-in heuristic). */
+int gg_set_word_bits(int bits);
+int gg_autotune(int nodes, int degree, int based_nodes, int groups, int batch, int reps,
+                int *best_bits, double *best_ms);
+int gg_tune_table_load(const char *path); /* returns entries loaded, or an error code */
+int gg_tune_table_save(const char *path);
+
+/* ---- Per-vertex distance features (eccentricity / distance histogram) from
+ * row popcounts of the same bit-parallel BFS.  hist is [nodes][maxd+1] int32:
+ * hist[v][k] = vertices at distance exactly k from v (k < maxd), hist[v][maxd]
+ * = vertices at distance >= maxd.  Adjacency in host memory. */
+int gg_distance_histogram(int nodes, int degree, const int32_t *adj, int maxd, int32_t *hist);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

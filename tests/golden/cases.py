@@ -1,0 +1,152 @@
+"""The golden evaluation cases: how each graph is built.
+
+Every case is rebuilt deterministically from its parameters (reference data
+files copied into tests/golden/data, or the package's seeded generators), so
+the committed CSV only stores inputs' names and the reference's outputs.
+"""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+
+from graphgolf_amd import graphs as G
+
+DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+GOLDEN_CSV = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden_eval.csv")
+
+
+def two_opt(edges: np.ndarray, nodes: int, rng: np.random.Generator, moves: int = 1) -> np.ndarray:
+    """Random 2-opt swaps (a,b),(c,d) -> (a,d),(c,b) keeping the graph simple."""
+    e = edges.copy()
+    present = set(map(tuple, np.sort(e, axis=1).tolist()))
+    done = 0
+    while done < moves:
+        i, j = rng.integers(0, len(e), 2)
+        if i == j:
+            continue
+        a, b = e[i]
+        c, d = e[j]
+        if len({a, b, c, d}) < 4:
+            continue
+        n1, n2 = tuple(sorted((a, d))), tuple(sorted((c, b)))
+        if n1 in present or n2 in present:
+            continue
+        present.discard(tuple(sorted((a, b))))
+        present.discard(tuple(sorted((c, d))))
+        present.add(n1)
+        present.add(n2)
+        e[i] = (a, d)
+        e[j] = (c, b)
+        done += 1
+    return e
+
+
+def c3_shapes(count: int, seed: int):
+    """(nodes, degree, seed) triples of the uneven batch: n ~ U[1024,16384],
+    d in {8,12,16,24,32}, n*d even."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        d = int(rng.choice([8, 12, 16, 24, 32]))
+        n = int(rng.integers(1024, 16385))
+        if (n * d) % 2:
+            n += 1
+        out.append((n, d, 1000 + k))
+    return out
+
+
+def build(case: dict):
+    """Returns (adj, kwargs) for a case: kwargs are based_nodes, groups,
+    added_centers (general) or the grid fields."""
+    kind = case["kind"]
+    if kind == "file":
+        e = G.read_edges(os.path.join(DATA, case["file"]))
+        n = int(e.max()) + 1
+        d = G.max_degree(n, e)  # meshes are not regular: short rows get self-loop padding
+        return G.edges_to_adj(n, d, e), dict(based_nodes=n, groups=1, added_centers=0)
+    if kind == "regular":
+        n, d = case["n"], case["d"]
+        e = G.random_regular(n, d, case["seed"])
+        if case.get("swaps"):
+            e = two_opt(e, n, np.random.default_rng(case["seed"]), case["swaps"])
+        c = case.get("c", 0)
+        g = case.get("g", 1)
+        return G.edges_to_adj(n, d, e), dict(based_nodes=(n - c) // g, groups=g, added_centers=c)
+    if kind == "symmetric":
+        n, d, g = case["n"], case["d"], case["g"]
+        e = G.random_symmetric(n, d, g, case["seed"])
+        return G.edges_to_adj(n, d, e), dict(based_nodes=n // g, groups=g, added_centers=0)
+    if kind == "disconnected":
+        n, d = case["n"], case["d"]
+        e1 = G.random_regular(n // 2, d, case["seed"])
+        e2 = G.random_regular(n // 2, d, case["seed"] + 1) + n // 2
+        e = np.concatenate([e1, e2])
+        return G.edges_to_adj(n, d, e), dict(based_nodes=n, groups=1, added_centers=0)
+    if kind in ("grid_file", "grid"):
+        if kind == "grid_file":
+            e, w, h = G.read_grid_edges(os.path.join(DATA, case["file"]))
+            n = w * h
+            d = 2 * len(e) // n
+        else:
+            w, h, d = case["w"], case["h"], case["d"]
+            n = w * h
+            e = G.random_grid(w, h, d, case["r"], case["seed"])
+        g = case.get("g", 1)
+        bh = h if g == 1 else h // 2
+        return G.edges_to_adj(n, d, e), dict(groups=g, based_nodes=n // g, height=h,
+                                             based_height=bh,
+                                             rotate_hash=G.rotate_hash(n, h, w, g))
+    raise ValueError(kind)
+
+
+def general_cases():
+    cases = []
+    for f in ["n10d3.random.edges", "n64d4.random.edges", "n128d4.random.edges",
+              "n256d4.random.edges", "n256d8.random.edges", "n512d4.random.edges",
+              "n114d29.random.edges", "n1024d10.random.edges", "n1726d30.random.edges",
+              "n9344d6.random.edges", "3x3", "4x4", "3x3x3", "4x4x4", "2x2x2x2", "3x3x3x3",
+              "w16h16d4.torus.edges", "w16h16d6.torus.edges", "w16h16d8.torus.edges"]:
+        for algo in (1, 2, 0):
+            cases.append(dict(id=f"file:{f}:a{algo}", kind="file", file=f, algo=algo))
+    # C1: n=256 d=8
+    for s in range(1, 4):
+        cases.append(dict(id=f"c1:{s}", kind="regular", n=256, d=8, seed=s, algo=1))
+    # C2: n=4096 d=16 groups=4
+    for s in (11, 12):
+        cases.append(dict(id=f"c2:{s}", kind="symmetric", n=4096, d=16, g=4, seed=s, algo=1))
+        cases.append(dict(id=f"c2bfs:{s}", kind="symmetric", n=4096, d=16, g=4, seed=s, algo=0))
+    # other symmetric factors, small
+    for g in (2, 3, 5, 8):
+        cases.append(dict(id=f"sym:g{g}", kind="symmetric", n=120 * g, d=6, g=g, seed=20 + g, algo=1))
+    # added centres (general/evaluation.c:205-245), with and without groups
+    cases.append(dict(id="ctr:g1", kind="regular", n=1000, d=8, seed=31, c=40, algo=1))
+    cases.append(dict(id="ctr:g4", kind="regular", n=1000, d=8, seed=32, c=16, g=4, algo=1))
+    cases.append(dict(id="ctr:g4:tiled", kind="regular", n=1000, d=8, seed=32, c=16, g=4, algo=2))
+    cases.append(dict(id="ctr:big", kind="regular", n=2600, d=10, seed=33, c=104, g=2, algo=1))
+    # C3 subset (uneven batch, one 2-opt swap each)
+    for (n, d, s) in c3_shapes(12, 7):
+        cases.append(dict(id=f"c3:{n}:{d}:{s}", kind="regular", n=n, d=d, seed=s, swaps=1, algo=1))
+    # disconnected
+    cases.append(dict(id="disc:1", kind="disconnected", n=512, d=4, seed=41, algo=1))
+    cases.append(dict(id="disc:0", kind="disconnected", n=512, d=4, seed=41, algo=0))
+    # C5 full size
+    cases.append(dict(id="c5:1", kind="regular", n=65536, d=16, seed=5, algo=1))
+    return cases
+
+
+def grid_cases():
+    cases = []
+    for f in ["grid_w5h5d4.random.edges", "grid_w10h10d4.random.edges",
+              "grid_w10h10d6.random.edges", "grid_w20h20d4.random.edges",
+              "grid_w100h100d4.random.edges"]:
+        for bfs in (False, True):
+            cases.append(dict(id=f"gfile:{f}:{int(bfs)}", kind="grid_file", file=f, bfs=bfs))
+    # C4: 100x100 d4 r2 seeded
+    cases.append(dict(id="c4:1", kind="grid", w=100, h=100, d=4, r=2, seed=1, bfs=False))
+    # relabelled (g=2, g=4) rotate_hash gathers on seeded grids
+    for g in (2, 4):
+        for bfs in (False, True):
+            cases.append(dict(id=f"grid:g{g}:{int(bfs)}", kind="grid", w=12, h=12, d=4, r=3,
+                              seed=50 + g, g=g, bfs=bfs))
+    return cases

@@ -1,0 +1,54 @@
+"""Writes tests/golden/golden_eval.csv from the reference's own evaluator.
+
+Runs in the development container only: it calls the reference's
+unmodified evaluation() (general/evaluation.c:375, grid/evaluation.c:192)
+compiled by oracle/Makefile into oracle/_ref/.  The CSV holds, per case,
+the reference's return value, diameter, ASPL (as a float hex string) and
+S = round(ASPL * n * (n-1)).
+
+    python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import csv
+import os
+import sys
+
+# The reference's OpenMP paths, loaded into this process next to MPICH, have
+# corrupted the heap here when run with several threads; the outputs do not
+# depend on the thread count, so the vectors are written single-threaded.
+os.environ["OMP_NUM_THREADS"] = "1"
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+
+import oracle  # noqa: E402
+from cases import GOLDEN_CSV, build, general_cases, grid_cases  # noqa: E402
+
+
+def main() -> None:
+    if not (oracle.ref_available("general") and oracle.ref_available("grid")):
+        raise SystemExit("oracle/_ref is not built (make -C oracle ref)")
+    rows = []
+    for case in general_cases():
+        adj, kw = build(case)
+        r = oracle.ref_evaluation(adj, kw["based_nodes"], kw["groups"], kw["added_centers"],
+                                  case["algo"])
+        rows.append((case["id"], int(r.ok), r.diameter, float(r.aspl).hex(), r.S))
+        print(rows[-1], flush=True)
+    for case in grid_cases():
+        adj, kw = build(case)
+        r = oracle.ref_grid_evaluation(adj, kw["groups"], kw["based_nodes"], kw["height"],
+                                       kw["based_height"], kw["rotate_hash"], case["bfs"])
+        aspl = float(r.aspl).hex() if r.ok else ""
+        rows.append((case["id"], int(r.ok), r.diameter, aspl, r.S))
+        print(rows[-1], flush=True)
+    with open(GOLDEN_CSV, "w", newline="") as f:
+        w = csv.writer(f)
+        w.writerow(["id", "ok", "diameter", "aspl_hex", "S"])
+        w.writerows(rows)
+
+
+if __name__ == "__main__":
+    main()
