@@ -93,6 +93,9 @@ def lib():
         "gg_plan_destroy": ([ctypes.c_void_p], None),
         "gg_set_word_bits": ([ctypes.c_int], ctypes.c_int),
         "gg_plan_kernel_ms": ([ctypes.c_void_p], ctypes.c_float),
+        "gg_plan_launch_timed": ([ctypes.c_void_p], ctypes.c_int),
+        "gg_trace_levels": ([ctypes.POINTER(GGGraph), ctypes.c_int, u64p,
+                             ctypes.POINTER(GGResult)], ctypes.c_int),
         "gg_synchronize": ([], ctypes.c_int),
         "gg_autotune": ([ctypes.c_int] * 6 + [ctypes.POINTER(ctypes.c_int),
                                               ctypes.POINTER(ctypes.c_double)], ctypes.c_int),

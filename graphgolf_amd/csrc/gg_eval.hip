@@ -16,9 +16,10 @@
 //     W-bit word per vertex (W = 8/16/32/64 sources).  n=65536 with W=16
 //     is 128 KiB, inside gfx950's 160 KiB LDS, so the level loop never
 //     touches HBM except for the adjacency stream.
-//   * The adjacency is converted once per call into a transposed, uint16,
-//     degree-padded table adjT[j][v] (2 B per entry, coalesced across lanes
-//     that own consecutive vertices); it is read from L2 by every tile.
+//   * The adjacency is converted once per call into a row-major uint16 table
+//     padded to whole 16-byte chunks (8 neighbour ids per load); it is read
+//     from L2 by every tile, with the next vertex's row prefetched while the
+//     current one gathers.
 //   * Direction-optimising levels.  While the frontier of the tile is small
 //     it is kept as an LDS queue of (vertex, new bits) and pushed to the
 //     neighbours with LDS atomic ORs (top-down, only for symmetric
@@ -52,14 +53,18 @@
 // Device-side descriptors.
 
 struct GraphDev {
-  const uint16_t *adjT;  // [dp][n] transposed, padded with the row's own id
+  const uint16_t *adjR;  // [n][dp] row-major, padded with the row's own id
   int32_t *flags;        // [0] = bad adjacency entries, [1] = asymmetric
   int32_t n, dp, split, c;
   int32_t push_ok, groups;
   int32_t hist_maxd;     // >0: accumulate per-vertex distance histogram
-  int32_t pad;
+  int32_t trace_units;   // units whose per-level timings are recorded
   int32_t *hist;         // [n][hist_maxd+1] (histogram mode only)
+  uint64_t *trace;       // [trace_units][kTraceLevels][2] or null
 };
+
+// Per-level trace record: (cycles since the unit started, mode word).
+constexpr int kTraceLevels = 64;
 
 struct UnitDesc {
   int32_t graph;
@@ -147,17 +152,20 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 
 // ---------------------------------------------------------------------------
 // Adjacency preparation: row-major int32 [n][d] (+ optional relabel) ->
-// transposed uint16 [dp][n].  Invalid entries are replaced by the row's own
-// id (memory-safe) and counted in flags[0]; flags[1] counts asymmetric
-// entries when check_sym is set.
+// row-major uint16 [n][dp], dp = d rounded up to 8 so that a row is a whole
+// number of 16-byte chunks (8 neighbour ids each).  Padding entries are the
+// row's own id (a self-loop adds nothing to a reach set).  Invalid entries
+// are replaced by the row's own id (memory-safe) and counted in flags[0];
+// flags[1] counts asymmetric entries when check_sym is set.
 
 __global__ void gg_prep_kernel(const int32_t *__restrict__ adj, const int32_t *__restrict__ relabel,
-                               int n, int d, int dp, uint16_t *__restrict__ adjT,
+                               int n, int d, int dp, uint16_t *__restrict__ adjR,
                                int32_t *__restrict__ flags, int check_sym) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int ri = relabel ? relabel[i] : i;
   const int32_t *row = adj + (size_t)i * d;
+  uint16_t *out = adjR + (size_t)ri * dp;
   int bad = 0, asym = 0;
   for (int j = 0; j < d; ++j) {
     int u = row[j];
@@ -177,9 +185,9 @@ __global__ void gg_prep_kernel(const int32_t *__restrict__ adj, const int32_t *_
         asym += (c1 != c2);
       }
     }
-    adjT[(size_t)j * n + ri] = (uint16_t)ru;
+    out[j] = (uint16_t)ru;
   }
-  for (int j = d; j < dp; ++j) adjT[(size_t)j * n + ri] = (uint16_t)ri;
+  for (int j = d; j < dp; ++j) out[j] = (uint16_t)ri;
   if (bad) atomicAdd(&flags[0], bad);
   if (asym) atomicAdd(&flags[1], asym);
 }
@@ -187,6 +195,21 @@ __global__ void gg_prep_kernel(const int32_t *__restrict__ adj, const int32_t *_
 // ---------------------------------------------------------------------------
 // The tile BFS.  One workgroup = one UnitDesc.  LDS layout (dynamic):
 //   R[n_cap] (W) | qb[2][qcap] (W) | qv[2][qcap] (u16) | red[2][16][4] | qcount[2]
+
+// 16 bytes of an adjacency row: 8 uint16 neighbour ids.
+typedef uint32_t Chunk __attribute__((ext_vector_type(4)));
+
+// OR of the reach words of 4 neighbour ids (two packed uint32 pairs).
+template <typename W> __device__ __forceinline__ W gather4(const W *R, uint32_t a, uint32_t b) {
+  return (W)(R[a & 0xffffu] | R[a >> 16] | R[b & 0xffffu] | R[b >> 16]);
+}
+
+// ORs a chunk's 8 neighbours into val, in two halves, stopping once full.
+template <typename W> __device__ __forceinline__ W gather_chunk(const W *R, Chunk c, W val, W full) {
+  val = (W)(val | gather4<W>(R, c.x, c.y));
+  if (val != full) val = (W)(val | gather4<W>(R, c.z, c.w));
+  return val;
+}
 
 template <typename W, int BLOCK, int VPT, bool HIST>
 __global__ void __launch_bounds__(BLOCK)
@@ -202,10 +225,11 @@ gg_tile_bfs_kernel(const GraphDev *__restrict__ graphs, const UnitDesc *__restri
   const UnitDesc U = units[uidx];
   const GraphDev G = graphs[U.graph];
   const int n = G.n, dp = G.dp, split = G.split;
+  const int nch = dp >> 3;  // 16-byte chunks per adjacency row
   const int c = n - split;
   // The descriptor's pointer is generic; say it is global so the gathers
   // compile to global_load (not flat_load).
-  const gptr<const uint16_t> adjT = (gptr<const uint16_t>)G.adjT;
+  const gptr<const Chunk> rows = (gptr<const Chunk>)G.adjR;
 
   size_t off = 0;
   W *R = (W *)(smem + off);
@@ -257,6 +281,8 @@ gg_tile_bfs_kernel(const GraphDev *__restrict__ graphs, const UnitDesc *__restri
 
   uint64_t acc_lo = 0, acc_hi = 0;
   int level = 0, done_level = n + 1, last_change = 0, disconnected = 0;
+  const bool tracing = G.trace != nullptr && uidx < G.trace_units;
+  const uint64_t t_start = tracing ? clock64() : 0;
   bool qvalid = push_ok && l <= qcap;
   uint32_t qn = (uint32_t)l;
   int cur = 0;
@@ -270,6 +296,9 @@ gg_tile_bfs_kernel(const GraphDev *__restrict__ graphs, const UnitDesc *__restri
     uint32_t st[StageRegs<W, VPT>::value];
     if (tid == 0) qcount[cur] = 0;  // this level reads qn (register); slot is next-next queue
 
+    // A push level records the next frontier only when it can plausibly fit
+    // the queue; otherwise the next level is a pull level anyway.
+    const bool build = do_push && qn * (uint32_t)dp <= 4u * (uint32_t)qcap;
     if (do_push) {
       // Top-down: push each frontier entry's new bits to its neighbours.
       const W *qb_c = cur ? qb1 : qb0;
@@ -277,54 +306,111 @@ gg_tile_bfs_kernel(const GraphDev *__restrict__ graphs, const UnitDesc *__restri
       W *qb_n = cur ? qb0 : qb1;
       uint16_t *qv_n = cur ? qv0 : qv1;
       uint32_t *qc_n = &qcount[cur ^ 1];
-      for (uint32_t e = tid; e < qn; e += BLOCK) {
-        const uint32_t u = qv_c[e];
-        const W f = qb_c[e];
-        for (int j = 0; j < dp; j += 4) {
-          uint32_t x[4];
+      // Wave-uniform trip count (the queue scan below needs every lane).
+      for (uint32_t e0 = (uint32_t)(tid - lane); e0 < qn; e0 += BLOCK) {
+        const uint32_t e = e0 + (uint32_t)lane;
+        const bool valid = e < qn;
+        const uint32_t u = valid ? (uint32_t)qv_c[e] : 0u;
+        const W f = valid ? qb_c[e] : (W)0;
+        for (int ch = 0; ch < nch; ++ch) {
+          Chunk cv = {0u, 0u, 0u, 0u};
+          if (valid) cv = rows[(size_t)u * nch + ch];
+          const uint32_t ids[8] = {cv.x & 0xffffu, cv.x >> 16, cv.y & 0xffffu, cv.y >> 16,
+                                   cv.z & 0xffffu, cv.z >> 16, cv.w & 0xffffu, cv.w >> 16};
+          // Eight independent LDS atomics first, then the bookkeeping.
+          W nbs[8];
 #pragma unroll
-          for (int t = 0; t < 4; ++t) x[t] = adjT[(size_t)(j + t) * n + u];
+          for (int k = 0; k < 8; ++k) nbs[k] = (W)0;
+          if (valid) {
 #pragma unroll
-          for (int t = 0; t < 4; ++t) {
-            const W old = lds_or<W>(R, x[t], f);
-            const W nb = (W)(f & (W)~old);
-            if (nb) {
-              const uint32_t pc = (uint32_t)wpopc<W>(nb);
-              if ((int)x[t] < split) p_lo += pc;
+            for (int k = 0; k < 8; ++k) nbs[k] = (W)(f & (W)~lds_or<W>(R, ids[k], f));
+          }
+          uint32_t mine = 0;
+#pragma unroll
+          for (int k = 0; k < 8; ++k) {
+            if (nbs[k]) {
+              const uint32_t pc = (uint32_t)wpopc<W>(nbs[k]);
+              if ((int)ids[k] < split) p_lo += pc;
               else p_hi += pc;
+              ++mine;
               if constexpr (HIST)
-                atomicAdd(&G.hist[(size_t)x[t] * (G.hist_maxd + 1) + min(level, G.hist_maxd)],
+                atomicAdd(&G.hist[(size_t)ids[k] * (G.hist_maxd + 1) + min(level, G.hist_maxd)],
                           (int)pc);
-              const uint32_t slot = atomicAdd(qc_n, 1u);
-              if (slot < (uint32_t)qcap) {
-                qb_n[slot] = nb;
-                qv_n[slot] = (uint16_t)x[t];
+            }
+          }
+          if (build) {
+            // One queue reservation per wave: exclusive scan of the lanes'
+            // counts, the last active lane reserves the total.
+            uint32_t incl = mine;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+              const uint32_t y = __shfl_up(incl, o, 64);
+              if (lane >= o) incl += y;
+            }
+            uint32_t base = 0;
+            if (lane == 63 && incl) base = atomicAdd(qc_n, incl);
+            base = __shfl(base, 63, 64);
+            uint32_t slot = base + incl - mine;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+              if (nbs[k]) {
+                if (slot < (uint32_t)qcap) {
+                  qb_n[slot] = nbs[k];
+                  qv_n[slot] = (uint16_t)ids[k];
+                }
+                ++slot;
               }
             }
           }
         }
       }
     } else {
-      // Bottom-up: every vertex gathers its neighbours' words (early exit on full).
-      gptr<const uint16_t> a = adjT + tid;
+      // Bottom-up: every vertex ORs its neighbours' words, stopping once its
+      // own word is full (tested every 4 ids).  The first two chunks (16 ids)
+      // of the rows of the next two vertices are in flight while the current
+      // vertex gathers.  `t` is laundered so that per-vertex addresses are
+      // recomputed inside this level instead of being hoisted out of the
+      // level loop (which would pin VPT of them in registers).
+      int t = tid;
+      asm volatile("" : "+v"(t));
+      const Chunk zero = {0u, 0u, 0u, 0u};
+      W o0 = (t < n) ? R[t] : full, o1 = full;
+      Chunk a0 = zero, b0 = zero, a1 = zero, b1 = zero;
+      if (o0 != full) {
+        a0 = rows[(size_t)t * nch];
+        if (nch > 1) b0 = rows[(size_t)t * nch + 1];
+      }
+      if (VPT > 1) {
+        const int v1 = t + BLOCK;
+        o1 = (v1 < n) ? R[v1] : full;
+        if (o1 != full) {
+          a1 = rows[(size_t)v1 * nch];
+          if (nch > 1) b1 = rows[(size_t)v1 * nch + 1];
+        }
+      }
 #pragma unroll
       for (int r = 0; r < VPT; ++r) {
-        const int v = r * BLOCK + tid;
-        W val = (W)0;
+        const int v = r * BLOCK + t;
+        const W old = o0;
+        const Chunk ca = a0, cb = b0;
+        o0 = o1;
+        a0 = a1;
+        b0 = b1;
+        if (r + 2 < VPT) {
+          const int v2 = v + 2 * BLOCK;
+          o1 = (v2 < n) ? R[v2] : full;
+          if (o1 != full) {
+            a1 = rows[(size_t)v2 * nch];
+            if (nch > 1) b1 = rows[(size_t)v2 * nch + 1];
+          }
+        }
+        W val = old;
         if (v < n) {
-          const W old = R[v];
-          val = old;
           if (val != full) {
-            gptr<const uint16_t> p = a;
-            for (int j = 0; j < dp; j += 4) {
-              const uint32_t i0 = p[0];
-              const uint32_t i1 = p[n];
-              const uint32_t i2 = p[2 * n];
-              const uint32_t i3 = p[3 * n];
-              val = (W)(val | R[i0] | R[i1] | R[i2] | R[i3]);
-              if (val == full) break;
-              p += 4 * n;
-            }
+            val = gather_chunk<W>(R, ca, val, full);
+            if (nch > 1 && val != full) val = gather_chunk<W>(R, cb, val, full);
+            for (int ch = 2; ch < nch && val != full; ++ch)
+              val = gather_chunk<W>(R, rows[(size_t)v * nch + ch], val, full);
           }
           const uint32_t pc = (uint32_t)wpopc<W>(val);
           if (v < split) p_lo += pc;
@@ -337,9 +423,9 @@ gg_tile_bfs_kernel(const GraphDev *__restrict__ graphs, const UnitDesc *__restri
           }
         }
         stage_put<W>(st, r, val);
-        a += BLOCK;
-        // Keep one running pointer instead of VPT precomputed addresses.
-        asm volatile("" : "+v"(a));
+        // Keep the scheduler from interleaving vertices (register pressure);
+        // the prefetches issued above stay in flight across this point.
+        __builtin_amdgcn_sched_barrier(0);
       }
     }
 
@@ -355,9 +441,11 @@ gg_tile_bfs_kernel(const GraphDev *__restrict__ graphs, const UnitDesc *__restri
     }
     __syncthreads();  // B1: all reads of R for this level are done
     if (!do_push) {
+      int t = tid;
+      asm volatile("" : "+v"(t));
 #pragma unroll
       for (int r = 0; r < VPT; ++r) {
-        const int v = r * BLOCK + tid;
+        const int v = r * BLOCK + t;
         if (v < n) R[v] = stage_get<W>(st, r);
       }
     }
@@ -376,7 +464,7 @@ gg_tile_bfs_kernel(const GraphDev *__restrict__ graphs, const UnitDesc *__restri
       tot_lo += L_lo;
       tot_hi += L_hi;
       chg = (L_lo + L_hi) != 0;
-      qvalid = qn_next <= (uint32_t)qcap;
+      qvalid = build && qn_next <= (uint32_t)qcap;
       qn = qn_next;
       cur ^= 1;
     } else {
@@ -386,6 +474,11 @@ gg_tile_bfs_kernel(const GraphDev *__restrict__ graphs, const UnitDesc *__restri
       qvalid = false;
     }
     if (chg) last_change = level;
+    if (tracing && tid == 0 && level < kTraceLevels) {
+      uint64_t *tr = G.trace + ((size_t)uidx * kTraceLevels + level) * 2;
+      tr[0] = clock64() - t_start;
+      tr[1] = (uint64_t)(do_push ? 1 : 0) | ((uint64_t)(do_push ? qn_next : L_lo + L_hi) << 1);
+    }
     const bool done = centre ? (tot_hi == hi_full) : (tot_lo + tot_hi == lo_full + hi_full);
     if (done) {
       done_level = level;
@@ -413,16 +506,20 @@ gg_tile_bfs_kernel(const GraphDev *__restrict__ graphs, const UnitDesc *__restri
 }
 
 // Per-graph combination of the unit partial sums
-// (general/evaluation.c:199-200, :241, :247-249 as integers).
-__global__ void gg_combine_kernel(const GraphMeta *__restrict__ meta, const UnitDesc *__restrict__ units,
-                                  const UnitOut *__restrict__ outs, GraphOut *__restrict__ res,
-                                  int ngraphs) {
-  const int gidx = blockIdx.x * blockDim.x + threadIdx.x;
+// (general/evaluation.c:199-200, :241, :247-249 as integers).  One
+// workgroup per graph; its units are contiguous.
+__global__ void __launch_bounds__(256)
+gg_combine_kernel(const GraphMeta *__restrict__ meta, const UnitDesc *__restrict__ units,
+                  const UnitOut *__restrict__ outs, GraphOut *__restrict__ res, int ngraphs) {
+  __shared__ uint64_t s64[3][256];
+  __shared__ int s32[4][256];
+  const int gidx = blockIdx.x;
+  const int tid = threadIdx.x;
   if (gidx >= ngraphs) return;
   const GraphMeta m = meta[gidx];
   uint64_t lo = 0, hi = 0, chi = 0;
   int diam = 1, ecc = 0, disc = 0, err = 0;
-  for (int k = 0; k < m.unit_count; ++k) {
+  for (int k = tid; k < m.unit_count; k += 256) {
     const int u = m.unit_begin + k;
     const UnitOut o = outs[u];
     if (units[u].centre) chi += o.acc_hi;
@@ -435,14 +532,36 @@ __global__ void gg_combine_kernel(const GraphMeta *__restrict__ meta, const Unit
     disc |= o.disconnected;
     err |= o.error;
   }
-  GraphOut r;
-  const uint64_t g = (uint64_t)m.groups, n = (uint64_t)m.n;
-  r.sum = lo * g + hi * 2ull * g + chi + n * (n - 1ull);
-  r.connected = disc ? 0 : 1;
-  r.diameter = disc ? m.n + 1 : diam;
-  r.max_ecc = ecc;
-  r.error = err;
-  res[gidx] = r;
+  s64[0][tid] = lo;
+  s64[1][tid] = hi;
+  s64[2][tid] = chi;
+  s32[0][tid] = diam;
+  s32[1][tid] = ecc;
+  s32[2][tid] = disc;
+  s32[3][tid] = err;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w) {
+      s64[0][tid] += s64[0][tid + w];
+      s64[1][tid] += s64[1][tid + w];
+      s64[2][tid] += s64[2][tid + w];
+      s32[0][tid] = max(s32[0][tid], s32[0][tid + w]);
+      s32[1][tid] = max(s32[1][tid], s32[1][tid + w]);
+      s32[2][tid] |= s32[2][tid + w];
+      s32[3][tid] |= s32[3][tid + w];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    GraphOut r;
+    const uint64_t g = (uint64_t)m.groups, n = (uint64_t)m.n;
+    r.sum = s64[0][0] * g + s64[1][0] * 2ull * g + s64[2][0] + n * (n - 1ull);
+    r.connected = s32[2][0] ? 0 : 1;
+    r.diameter = s32[2][0] ? m.n + 1 : s32[0][0];
+    r.max_ecc = s32[1][0];
+    r.error = s32[3][0];
+    res[gidx] = r;
+  }
 }
 
 // A tiny fill kernel (captured graphs must not record memset nodes).
@@ -732,7 +851,7 @@ int assemble(int count, const gg_graph *gs, int hist_maxd, Assembly &A) {
     Slot &s = A.slots[i];
     s.n = g.nodes;
     s.d = g.degree;
-    s.dp = (g.degree + 3) & ~3;
+    s.dp = (g.degree + 7) & ~7;
     s.b = g.based_nodes;
     s.g = g.groups;
     s.c = g.added_centers;
@@ -804,7 +923,8 @@ struct DevSet {
   }
 };
 
-int prepare_buffers(Assembly &A, DevSet &D, int hist_maxd, int32_t *hist_dev) {
+int prepare_buffers(Assembly &A, DevSet &D, int hist_maxd, int32_t *hist_dev,
+                    uint64_t *trace_dev = nullptr, int trace_units = 0) {
   const int count = (int)A.slots.size();
   int rc;
   if ((rc = D.adj_in.ensure(A.adj_elems * 4))) return rc;
@@ -820,7 +940,7 @@ int prepare_buffers(Assembly &A, DevSet &D, int hist_maxd, int32_t *hist_dev) {
   for (int i = 0; i < count; ++i) {
     const Slot &s = A.slots[i];
     GraphDev &g = A.gdev[i];
-    g.adjT = (const uint16_t *)D.adjT.p + s.adjT_off;
+    g.adjR = (const uint16_t *)D.adjT.p + s.adjT_off;
     g.flags = (int32_t *)D.flags.p + 2 * i;
     g.n = s.n;
     g.dp = s.dp;
@@ -829,8 +949,9 @@ int prepare_buffers(Assembly &A, DevSet &D, int hist_maxd, int32_t *hist_dev) {
     g.push_ok = 1;
     g.groups = s.g;
     g.hist_maxd = hist_maxd;
-    g.pad = 0;
+    g.trace_units = trace_units;
     g.hist = hist_dev;
+    g.trace = trace_dev;
   }
   GG_HIP(hipMemcpyAsync(D.gdesc.p, A.gdev.data(), count * sizeof(GraphDev), hipMemcpyHostToDevice, E.stream));
   GG_HIP(hipMemcpyAsync(D.units.p, A.units.data(), A.units.size() * sizeof(UnitDesc),
@@ -868,7 +989,7 @@ int enqueue(Assembly &A, DevSet &D, const int32_t *const *adj_src, const int32_t
                        lg.unit_begin, lg.n_cap, lg.qcap);
   }
   if (ev_end) GG_HIP(hipEventRecord(ev_end, E.stream));
-  hipLaunchKernelGGL(gg_combine_kernel, dim3((count + 127) / 128), dim3(128), 0, E.stream,
+  hipLaunchKernelGGL(gg_combine_kernel, dim3(count), dim3(256), 0, E.stream,
                      (const GraphMeta *)D.meta.p, (const UnitDesc *)D.units.p,
                      (const UnitOut *)D.outs.p, (GraphOut *)D.gout.p, count);
   GG_HIP(hipGetLastError());
@@ -906,6 +1027,9 @@ int check_permutation(const int32_t *p, int n) {
   return GG_OK;
 }
 
+static int g_trace_units = 0;      // >0: next eval_host records per-level traces
+static uint64_t *g_trace_host = nullptr;
+
 int eval_host(int count, const gg_graph *gs, gg_result *out, bool device_ptrs, int hist_maxd,
               int32_t *hist_host) {
   // Argument checks first: they need no device.
@@ -927,7 +1051,15 @@ int eval_host(int count, const gg_graph *gs, gg_result *out, bool device_ptrs, i
     hist_dev = (int32_t *)D.hist.p;
     GG_HIP(hipMemsetAsync(hist_dev, 0, hist_bytes, E.stream));
   }
-  if ((rc = prepare_buffers(A, D, hist_maxd, hist_dev))) return rc;
+  DevBuf trace_buf;
+  const int trace_units = std::min<int>(g_trace_units, (int)A.units.size());
+  if (trace_units > 0) {
+    size_t tb = (size_t)A.units.size() * kTraceLevels * 2 * sizeof(uint64_t);
+    if ((rc = trace_buf.ensure(tb))) return rc;
+    GG_HIP(hipMemsetAsync(trace_buf.p, 0, tb, E.stream));
+  }
+  if ((rc = prepare_buffers(A, D, hist_maxd, hist_dev, (uint64_t *)trace_buf.p, trace_units)))
+    return rc;
   std::vector<const int32_t *> src(count), rel(count);
   std::vector<int> sym(count);
   for (int i = 0; i < count; ++i) {
@@ -951,7 +1083,13 @@ int eval_host(int count, const gg_graph *gs, gg_result *out, bool device_ptrs, i
   if ((rc = enqueue(A, D, src.data(), rel.data(), sym.data()))) return rc;
   if (hist_maxd > 0)
     GG_HIP(hipMemcpyAsync(hist_host, hist_dev, hist_bytes, hipMemcpyDeviceToHost, E.stream));
-  return collect(A, D, out);
+  if (trace_units > 0)
+    GG_HIP(hipMemcpyAsync(g_trace_host, trace_buf.p,
+                          (size_t)trace_units * kTraceLevels * 2 * sizeof(uint64_t),
+                          hipMemcpyDeviceToHost, E.stream));
+  rc = collect(A, D, out);
+  trace_buf.release();
+  return rc;
 }
 
 // ---- Plans ---------------------------------------------------------------
@@ -963,8 +1101,9 @@ struct Plan {
   std::vector<int> sym;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  hipEvent_t ev_begin = nullptr, ev_end = nullptr;  // around the tile-BFS launches
-  bool launched = false;
+  hipEvent_t ev_begin = nullptr, ev_end = nullptr;    // around the tile-BFS launches (graph)
+  hipEvent_t tev_begin = nullptr, tev_end = nullptr;  // same, for eager timed launches
+  bool launched = false, timed = false;
 };
 
 }  // namespace gg
@@ -1009,6 +1148,16 @@ int gg_distance_histogram(int nodes, int degree, const int32_t *adj, int maxd, i
   gg_graph g = {nodes, degree, nodes, 1, 0, 0, adj, nullptr};
   gg_result r;
   return eval_host(1, &g, &r, false, maxd, hist);
+}
+
+int gg_trace_levels(const gg_graph *g, int units, uint64_t *trace, gg_result *out) {
+  if (!g || !trace || !out || units < 1) return set_error(GG_ERR_INVALID, "bad trace arguments");
+  g_trace_units = units;
+  g_trace_host = trace;
+  int rc = eval_host(1, g, out, false, 0, nullptr);
+  g_trace_units = 0;
+  g_trace_host = nullptr;
+  return rc;
 }
 
 int gg_set_word_bits(int bits) {
@@ -1149,7 +1298,8 @@ gg_plan *gg_plan_create(int count, const gg_graph *shapes) {
     delete p;
     return nullptr;
   }
-  if (hipEventCreate(&p->ev_begin) != hipSuccess || hipEventCreate(&p->ev_end) != hipSuccess) {
+  if (hipEventCreate(&p->ev_begin) != hipSuccess || hipEventCreate(&p->ev_end) != hipSuccess ||
+      hipEventCreate(&p->tev_begin) != hipSuccess || hipEventCreate(&p->tev_end) != hipSuccess) {
     set_error(GG_ERR_HIP, "event creation failed");
     p->D.release();
     delete p;
@@ -1188,6 +1338,7 @@ int gg_plan_launch(gg_plan *p) {
   if (!p || !p->exec) return set_error(GG_ERR_STATE, "plan not ready");
   GG_HIP(hipGraphLaunch(p->exec, engine().stream));
   p->launched = true;
+  p->timed = false;
   return GG_OK;
 }
 
@@ -1203,10 +1354,24 @@ int gg_plan_run(gg_plan *p, gg_result *out) {
   return gg_plan_wait(p, out);
 }
 
+// The plan's kernels enqueued directly on the stream (not through the
+// captured graph), with HIP events around the tile-BFS launches so their
+// duration can be read by gg_plan_kernel_ms() after gg_plan_wait().
+int gg_plan_launch_timed(gg_plan *p) {
+  if (!p || !p->exec) return set_error(GG_ERR_STATE, "plan not ready");
+  int rc = enqueue(p->A, p->D, p->src.data(), p->rel.data(), p->sym.data(), p->tev_begin,
+                   p->tev_end);
+  if (rc) return rc;
+  p->launched = true;
+  p->timed = true;
+  return GG_OK;
+}
+
 float gg_plan_kernel_ms(gg_plan *p) {
-  if (!p || !p->ev_begin) return -1.0f;
+  if (!p) return -1.0f;
   float ms = -1.0f;
-  if (hipEventElapsedTime(&ms, p->ev_begin, p->ev_end) != hipSuccess) return -1.0f;
+  hipEvent_t b = p->timed ? p->tev_begin : p->ev_begin, e = p->timed ? p->tev_end : p->ev_end;
+  if (!b || hipEventElapsedTime(&ms, b, e) != hipSuccess) return -1.0f;
   return ms;
 }
 
@@ -1222,8 +1387,8 @@ void gg_plan_destroy(gg_plan *p) {
   hipStreamSynchronize(engine().stream);
   if (p->exec) hipGraphExecDestroy(p->exec);
   if (p->graph) hipGraphDestroy(p->graph);
-  if (p->ev_begin) hipEventDestroy(p->ev_begin);
-  if (p->ev_end) hipEventDestroy(p->ev_end);
+  for (hipEvent_t ev : {p->ev_begin, p->ev_end, p->tev_begin, p->tev_end})
+    if (ev) hipEventDestroy(ev);
   p->D.release();
   delete p;
 }

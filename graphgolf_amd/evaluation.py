@@ -141,6 +141,10 @@ class Plan:
     def launch(self) -> None:
         L.check(L.lib().gg_plan_launch(self._p))
 
+    def launch_timed(self) -> None:
+        """Enqueue the same kernels eagerly with HIP events around the tile BFS."""
+        L.check(L.lib().gg_plan_launch_timed(self._p))
+
     def wait(self) -> list:
         L.check(L.lib().gg_plan_wait(self._p, self._out))
         self._keep = self._keep[: self.n]
@@ -160,6 +164,19 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+def trace_levels(adj, units: int = 8, based_nodes=None, groups=1, added_centers=0,
+                 symmetric=False):
+    """Per-level shader-cycle timings of the first `units` source tiles of one
+    evaluation: array [units, 64, 2] of (cycles since start, mode word)."""
+    adj = L.c32(adj)
+    g = _graph(adj, based_nodes, groups, added_centers, None, symmetric)
+    out = np.zeros((units, 64, 2), dtype=np.uint64)
+    r = L.GGResult()
+    L.check(L.lib().gg_trace_levels(ctypes.byref(g), units,
+                                    out.ctypes.data_as(L.u64p), ctypes.byref(r)))
+    return out, _result(r, adj.shape[0])
 
 
 def synchronize() -> None:

@@ -98,8 +98,11 @@ int gg_plan_run(gg_plan *p, gg_result *out);        /* replay + wait + results *
 int gg_plan_launch(gg_plan *p);                     /* replay only (async) */
 int gg_plan_wait(gg_plan *p, gg_result *out);       /* wait for the last launch */
 void gg_plan_destroy(gg_plan *p);
+/* Enqueues the plan's kernels directly (not through the graph) with HIP
+ * events around the tile-BFS launches; complete it with gg_plan_wait(). */
+int gg_plan_launch_timed(gg_plan *p);
 /* Duration (ms, HIP events on the evaluator's stream) of the tile-BFS
- * kernels in the plan's last completed replay; -1 if unavailable. */
+ * kernels in the plan's last completed launch; -1 if unavailable. */
 float gg_plan_kernel_ms(gg_plan *p);
 /* Waits for all work of the evaluator's stream and the device. */
 int gg_synchronize(void);
@@ -118,6 +121,13 @@ int gg_tune_table_save(const char *path);
  * hist[v][k] = vertices at distance exactly k from v (k < maxd), hist[v][maxd]
  * = vertices at distance >= maxd.  Adjacency in host memory. */
 int gg_distance_histogram(int nodes, int degree, const int32_t *adj, int maxd, int32_t *hist);
+
+/* ---- Profiling aid: one evaluation that records, for the first `units`
+ * source tiles, per BFS level k (k < 64) the shader-clock cycles since the
+ * tile started and a mode word (bit 0: push level; bits 1..: queue length
+ * after a push level, or the reached-pair count after a pull level):
+ * trace[(unit*64 + k)*2 + {0,1}]. */
+int gg_trace_levels(const gg_graph *g, int units, uint64_t *trace, gg_result *out);
 
 #ifdef __cplusplus
 }
