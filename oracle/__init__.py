@@ -228,3 +228,53 @@ def ref_rotate_hash(nodes, height, width, groups) -> np.ndarray:
     L.create_rotate_hash.argtypes = [ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_int)]
     L.create_rotate_hash(nodes, height, width, groups, _ip(h))
     return h
+
+
+def ref_sa_general(edges, nodes, degree, seed, ncalcs, max_temp=100.0, min_temp=0.217147,
+                   cooling_cycle=1, hill_climbing=False):
+    """Run the reference's sa() (general/sa.c:413-520) from srandom(seed) on a
+    copy of `edges`.  Returns (best_edges, best_diam, best_aspl, steps)."""
+    import math
+
+    L = ref_lib("general")
+    e = np.ascontiguousarray(edges, dtype=np.int32).copy()
+    lines = e.shape[0]
+    # lower bound (general/main.c:244-265) for the "found optimum" stop
+    diam_lb, n_lb, r = -1, 1, 1
+    aspl_lb = 0.0
+    while True:
+        tmp = n_lb + degree * int(math.pow(degree - 1, r - 1))
+        if tmp >= nodes:
+            break
+        n_lb = tmp
+        aspl_lb += r * degree * math.pow(degree - 1, r - 1)
+        diam_lb = r
+        r += 1
+    diam_lb += 1
+    aspl_lb += diam_lb * (nodes - n_lb)
+    aspl_lb /= (nodes - 1)
+    if hill_climbing:
+        max_temp = min_temp = 0.0
+        rate = 1.0
+    else:
+        rate = math.pow(min_temp / max_temp, cooling_cycle / ncalcs) if max_temp != min_temp else 1.0
+    L.sa.restype = ctypes.c_longlong
+    L.sa.argtypes = [ctypes.c_int] * 4 + [ctypes.c_double, ctypes.c_longlong, ctypes.c_double,
+                                          ctypes.c_int, ctypes.c_double, ctypes.c_bool,
+                                          ctypes.c_bool, ctypes.POINTER(ctypes.c_double),
+                                          ctypes.POINTER(ctypes.c_int),
+                                          ctypes.POINTER(ctypes.c_int),
+                                          ctypes.POINTER(ctypes.c_double), ctypes.c_int,
+                                          ctypes.c_int, ctypes.c_int,
+                                          ctypes.POINTER(ctypes.c_longlong), ctypes.c_bool,
+                                          ctypes.c_int]
+    L.ref_srandom.argtypes = [ctypes.c_uint]
+    mde = ctypes.c_double(0)
+    diam = ctypes.c_int(0)
+    aspl = ctypes.c_double(0)
+    tot = ctypes.c_longlong(0)
+    L.ref_srandom(seed)
+    steps = L.sa(nodes, lines, degree, 1, max_temp, ncalcs, rate, diam_lb, aspl_lb,
+                 bool(hill_climbing), False, ctypes.byref(mde), _ip(e), ctypes.byref(diam),
+                 ctypes.byref(aspl), cooling_cycle, 0, nodes, ctypes.byref(tot), True, 1)
+    return e, diam.value, aspl.value, int(steps)

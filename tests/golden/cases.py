@@ -14,6 +14,7 @@ from graphgolf_amd import graphs as G
 
 DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
 GOLDEN_CSV = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden_eval.csv")
+GOLDEN_SA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden_sa.json")
 
 
 def two_opt(edges: np.ndarray, nodes: int, rng: np.random.Generator, moves: int = 1) -> np.ndarray:
@@ -150,3 +151,15 @@ def grid_cases():
             cases.append(dict(id=f"grid:g{g}:{int(bfs)}", kind="grid", w=12, h=12, d=4, r=3,
                               seed=50 + g, g=g, bfs=bfs))
     return cases
+
+
+def sa_cases():
+    """SA runs pinned against the reference's sa(): C1 (n=256, d=8, 10k
+    steps, T 100 -> 0.217147) and smaller variants."""
+    return [
+        dict(id="sa:c1:a", n=256, d=8, graph_seed=1, seed=7, ncalcs=10000),
+        dict(id="sa:c1:b", n=256, d=8, graph_seed=2, seed=11, ncalcs=10000),
+        dict(id="sa:n512d4", n=512, d=4, graph_seed=3, seed=1, ncalcs=3000),
+        dict(id="sa:hill", n=128, d=6, graph_seed=4, seed=5, ncalcs=2000, hill=True),
+        dict(id="sa:cycle5", n=200, d=5, graph_seed=5, seed=9, ncalcs=3000, cooling_cycle=5),
+    ]

@@ -4,13 +4,16 @@ Runs in the development container only: it calls the reference's
 unmodified evaluation() (general/evaluation.c:375, grid/evaluation.c:192)
 compiled by oracle/Makefile into oracle/_ref/.  The CSV holds, per case,
 the reference's return value, diameter, ASPL (as a float hex string) and
-S = round(ASPL * n * (n-1)).
+S = round(ASPL * n * (n-1)); and golden_sa.json, the outcome of the
+reference's sa() (general/sa.c:413-520) from fixed seeds.
 
     python tests/golden/make_golden.py
 """
 from __future__ import annotations
 
 import csv
+import hashlib
+import json
 import os
 import sys
 
@@ -24,7 +27,8 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
 
 import oracle  # noqa: E402
-from cases import GOLDEN_CSV, build, general_cases, grid_cases  # noqa: E402
+from cases import GOLDEN_CSV, GOLDEN_SA, build, general_cases, grid_cases, sa_cases  # noqa: E402
+from graphgolf_amd import graphs as G  # noqa: E402
 
 
 def main() -> None:
@@ -48,6 +52,18 @@ def main() -> None:
         w = csv.writer(f)
         w.writerow(["id", "ok", "diameter", "aspl_hex", "S"])
         w.writerows(rows)
+    # Simulated-annealing trajectories of the reference's sa() (general/sa.c:413).
+    sa_rows = []
+    for case in sa_cases():
+        e0 = G.random_regular(case["n"], case["d"], case["graph_seed"])
+        e, diam, aspl, steps = oracle.ref_sa_general(
+            e0, case["n"], case["d"], case["seed"], case["ncalcs"],
+            cooling_cycle=case.get("cooling_cycle", 1), hill_climbing=case.get("hill", False))
+        sa_rows.append(dict(case, best_diameter=diam, best_aspl_hex=float(aspl).hex(), steps=steps,
+                            edges_sha256=hashlib.sha256(e.tobytes()).hexdigest()))
+        print(sa_rows[-1], flush=True)
+    with open(GOLDEN_SA, "w") as f:
+        json.dump(sa_rows, f, indent=1)
 
 
 if __name__ == "__main__":
