@@ -192,6 +192,41 @@ __global__ void gg_prep_kernel(const int32_t *__restrict__ adj, const int32_t *_
   if (asym) atomicAdd(&flags[1], asym);
 }
 
+// Patched variant for batches of proposals that differ from one base graph
+// in a few adjacency entries: slot y = blockIdx.y reads the shared base table
+// and overrides the entries listed in patches[y][k] = (v*d + j, new id)
+// (index -1 = unused).  The caller guarantees symmetry (SA moves keep it).
+__global__ void gg_prep_patched_kernel(const int32_t *__restrict__ base,
+                                       const int32_t *__restrict__ relabel,
+                                       const int32_t *__restrict__ patches, int max_patch, int n,
+                                       int d, int dp, uint16_t *__restrict__ adjR0,
+                                       size_t slot_stride, int32_t *__restrict__ flags0) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int slot = blockIdx.y;
+  if (i >= n) return;
+  const int32_t *P = patches + (size_t)slot * max_patch * 2;
+  const int ri = relabel ? relabel[i] : i;
+  const int32_t *row = base + (size_t)i * d;
+  uint16_t *out = adjR0 + slot * slot_stride + (size_t)ri * dp;
+  int bad = 0;
+  for (int j = 0; j < d; ++j) {
+    int u = row[j];
+    const int at = i * d + j;
+    for (int k = 0; k < max_patch; ++k)
+      if (P[2 * k] == at) u = P[2 * k + 1];
+    int ru;
+    if (u < 0 || u >= n) {
+      ++bad;
+      ru = ri;
+    } else {
+      ru = relabel ? relabel[u] : u;
+    }
+    out[j] = (uint16_t)ru;
+  }
+  for (int j = d; j < dp; ++j) out[j] = (uint16_t)ri;
+  if (bad) atomicAdd(&flags0[2 * slot], bad);
+}
+
 // ---------------------------------------------------------------------------
 // The tile BFS.  One workgroup = one UnitDesc.  LDS layout (dynamic):
 //   R[n_cap] (W) | qb[2][qcap] (W) | qv[2][qcap] (u16) | red[2][16][4] | qcount[2]
@@ -964,20 +999,36 @@ int prepare_buffers(Assembly &A, DevSet &D, int hist_maxd, int32_t *hist_dev,
 // Enqueue prep + tile BFS + combine for an assembled set whose adjacency
 // already sits in `adj_src[i]` (device pointers) -- graph-capturable (no
 // memset nodes, no host sync).
+struct PatchSrc {
+  const int32_t *base;
+  const int32_t *patches;
+  int max_patch;
+};
+
 int enqueue(Assembly &A, DevSet &D, const int32_t *const *adj_src, const int32_t *const *rel_src,
-            const int *check_sym, hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr) {
+            const int *check_sym, hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr,
+            const PatchSrc *patched = nullptr) {
   const int count = (int)A.slots.size();
   // flags cleared by a kernel-free path: hipMemsetAsync is avoided so the
   // sequence can be captured (a small fill is folded into the prep launch
   // ordering below through a dedicated clear kernel).
   hipLaunchKernelGGL(gg_clear_i32_kernel, dim3((2 * count + 255) / 256), dim3(256), 0, E.stream,
                      (int32_t *)D.flags.p, 2 * count);
-  for (int i = 0; i < count; ++i) {
-    const Slot &s = A.slots[i];
-    hipLaunchKernelGGL(gg_prep_kernel, dim3((s.n + 255) / 256), dim3(256), 0, E.stream,
-                       adj_src[i], rel_src[i], s.n, s.d, s.dp,
-                       (uint16_t *)D.adjT.p + s.adjT_off, (int32_t *)D.flags.p + 2 * i,
-                       check_sym[i]);
+  if (patched) {
+    // All slots share one shape (checked at plan creation) and one base table.
+    const Slot &s = A.slots[0];
+    const size_t stride = count > 1 ? A.slots[1].adjT_off - A.slots[0].adjT_off : 0;
+    hipLaunchKernelGGL(gg_prep_patched_kernel, dim3((s.n + 255) / 256, count), dim3(256), 0, E.stream,
+                       patched->base, rel_src[0], patched->patches, patched->max_patch, s.n, s.d,
+                       s.dp, (uint16_t *)D.adjT.p, stride, (int32_t *)D.flags.p);
+  } else {
+    for (int i = 0; i < count; ++i) {
+      const Slot &s = A.slots[i];
+      hipLaunchKernelGGL(gg_prep_kernel, dim3((s.n + 255) / 256), dim3(256), 0, E.stream,
+                         adj_src[i], rel_src[i], s.n, s.d, s.dp,
+                         (uint16_t *)D.adjT.p + s.adjT_off, (int32_t *)D.flags.p + 2 * i,
+                         check_sym[i]);
+    }
   }
   if (ev_begin) GG_HIP(hipEventRecord(ev_begin, E.stream));
   for (const LaunchGroup &lg : A.groups) {
@@ -1097,6 +1148,10 @@ int eval_host(int count, const gg_graph *gs, gg_result *out, bool device_ptrs, i
 struct Plan {
   Assembly A;
   DevSet D;
+  // Patched plans: every slot is the base graph plus up to max_patch entries.
+  int max_patch = 0;
+  DevBuf base, patches;
+  PatchSrc psrc{nullptr, nullptr, 0};
   std::vector<const int32_t *> src, rel;
   std::vector<int> sym;
   hipGraph_t graph = nullptr;
@@ -1257,20 +1312,27 @@ int gg_tune_table_load(const char *path) {
   return loaded;
 }
 
-gg_plan *gg_plan_create(int count, const gg_graph *shapes) {
+static gg_plan *plan_fail(gg_plan *p) {
+  for (hipEvent_t ev : {p->ev_begin, p->ev_end, p->tev_begin, p->tev_end})
+    if (ev) hipEventDestroy(ev);
+  p->D.release();
+  p->base.release();
+  p->patches.release();
+  delete p;
+  return nullptr;
+}
+
+static gg_plan *plan_create(int count, const gg_graph *shapes, int max_patch) {
   if (gg::init(-1)) return nullptr;
-  if (count <= 0 || !shapes) {
+  if (count <= 0 || !shapes || max_patch < 0 || max_patch > 64) {
     set_error(GG_ERR_INVALID, "bad plan arguments");
     return nullptr;
   }
   for (int i = 0; i < count; ++i)
     if (shapes[i].relabel && check_permutation(shapes[i].relabel, shapes[i].nodes)) return nullptr;
   gg_plan *p = new gg_plan();
-  if (assemble(count, shapes, 0, p->A) || prepare_buffers(p->A, p->D, 0, nullptr)) {
-    p->D.release();
-    delete p;
-    return nullptr;
-  }
+  p->max_patch = max_patch;
+  if (assemble(count, shapes, 0, p->A) || prepare_buffers(p->A, p->D, 0, nullptr)) return plan_fail(p);
   p->src.resize(count);
   p->rel.assign(count, nullptr);
   p->sym.resize(count);
@@ -1282,42 +1344,87 @@ gg_plan *gg_plan_create(int count, const gg_graph *shapes) {
       if (hipMemcpyAsync(rd, shapes[i].relabel, (size_t)shapes[i].nodes * 4, hipMemcpyHostToDevice,
                          engine().stream) != hipSuccess) {
         set_error(GG_ERR_HIP, "relabel upload failed");
-        p->D.release();
-        delete p;
-        return nullptr;
+        return plan_fail(p);
       }
       p->rel[i] = rd;
     }
+  }
+  if (max_patch > 0) {
+    const gg_graph &g = shapes[0];
+    std::vector<int32_t> none((size_t)count * max_patch * 2, -1);
+    if (p->base.ensure((size_t)g.nodes * g.degree * 4) ||
+        p->patches.ensure(none.size() * 4) ||
+        hipMemcpyAsync(p->patches.p, none.data(), none.size() * 4, hipMemcpyHostToDevice,
+                       engine().stream) != hipSuccess)
+      return plan_fail(p);
+    if (hipStreamSynchronize(engine().stream) != hipSuccess) return plan_fail(p);
+    p->psrc = {(const int32_t *)p->base.p, (const int32_t *)p->patches.p, max_patch};
+  }
+  if (hipEventCreate(&p->ev_begin) != hipSuccess || hipEventCreate(&p->ev_end) != hipSuccess ||
+      hipEventCreate(&p->tev_begin) != hipSuccess || hipEventCreate(&p->tev_end) != hipSuccess) {
+    set_error(GG_ERR_HIP, "event creation failed");
+    return plan_fail(p);
   }
   // Capture prep + BFS + combine + result copy once; replays reuse it.
   hipStream_t s = engine().stream;
   if (hipStreamSynchronize(s) != hipSuccess ||
       hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
     set_error(GG_ERR_HIP, "stream capture could not start");
-    p->D.release();
-    delete p;
-    return nullptr;
+    return plan_fail(p);
   }
-  if (hipEventCreate(&p->ev_begin) != hipSuccess || hipEventCreate(&p->ev_end) != hipSuccess ||
-      hipEventCreate(&p->tev_begin) != hipSuccess || hipEventCreate(&p->tev_end) != hipSuccess) {
-    set_error(GG_ERR_HIP, "event creation failed");
-    p->D.release();
-    delete p;
-    return nullptr;
-  }
-  int rc = enqueue(p->A, p->D, p->src.data(), p->rel.data(), p->sym.data(), p->ev_begin, p->ev_end);
+  int rc = enqueue(p->A, p->D, p->src.data(), p->rel.data(), p->sym.data(), p->ev_begin, p->ev_end,
+                   max_patch > 0 ? &p->psrc : nullptr);
   hipGraph_t graph = nullptr;
   hipError_t e = hipStreamEndCapture(s, &graph);
   if (rc || e != hipSuccess || !graph ||
       hipGraphInstantiate(&p->exec, graph, nullptr, nullptr, 0) != hipSuccess) {
     if (!rc) set_error(GG_ERR_HIP, "graph capture/instantiate failed: %s", hipGetErrorString(e));
     if (graph) hipGraphDestroy(graph);
-    p->D.release();
-    delete p;
-    return nullptr;
+    return plan_fail(p);
   }
   p->graph = graph;
   return p;
+}
+
+gg_plan *gg_plan_create(int count, const gg_graph *shapes) { return plan_create(count, shapes, 0); }
+
+gg_plan *gg_plan_create_patched(int count, const gg_graph *shape, int max_patch) {
+  if (!shape || count <= 0 || max_patch < 1) {
+    set_error(GG_ERR_INVALID, "bad patched plan arguments");
+    return nullptr;
+  }
+  if (!(shape->flags & GG_GRAPH_SYMMETRIC)) {
+    set_error(GG_ERR_INVALID, "patched plans need GG_GRAPH_SYMMETRIC (moves keep symmetry)");
+    return nullptr;
+  }
+  std::vector<gg_graph> shapes(count, *shape);
+  return plan_create(count, shapes.data(), max_patch);
+}
+
+int gg_plan_set_base(gg_plan *p, const int32_t *adj_host) {
+  if (!p || !p->max_patch || !adj_host) return set_error(GG_ERR_INVALID, "not a patched plan");
+  const Slot &s = p->A.slots[0];
+  GG_HIP(hipMemcpyAsync(p->base.p, adj_host, (size_t)s.n * s.d * 4, hipMemcpyHostToDevice,
+                        engine().stream));
+  return GG_OK;
+}
+
+int32_t *gg_plan_base_device(gg_plan *p) {
+  return (p && p->max_patch) ? (int32_t *)p->base.p : nullptr;
+}
+
+int gg_plan_set_patches(gg_plan *p, const int32_t *patches_host) {
+  if (!p || !p->max_patch || !patches_host) return set_error(GG_ERR_INVALID, "not a patched plan");
+  const Slot &s = p->A.slots[0];
+  const size_t cnt = p->A.slots.size() * (size_t)p->max_patch;
+  for (size_t k = 0; k < cnt; ++k) {
+    const int32_t at = patches_host[2 * k], val = patches_host[2 * k + 1];
+    if (at != -1 && (at < 0 || at >= s.n * s.d || val < 0 || val >= s.n))
+      return set_error(GG_ERR_INVALID, "patch %zu (%d -> %d) out of range", k, at, val);
+  }
+  GG_HIP(hipMemcpyAsync(p->patches.p, patches_host, cnt * 2 * 4, hipMemcpyHostToDevice,
+                        engine().stream));
+  return GG_OK;
 }
 
 int gg_plan_set_adj(gg_plan *p, int index, const int32_t *adj_host) {
@@ -1360,7 +1467,7 @@ int gg_plan_run(gg_plan *p, gg_result *out) {
 int gg_plan_launch_timed(gg_plan *p) {
   if (!p || !p->exec) return set_error(GG_ERR_STATE, "plan not ready");
   int rc = enqueue(p->A, p->D, p->src.data(), p->rel.data(), p->sym.data(), p->tev_begin,
-                   p->tev_end);
+                   p->tev_end, p->max_patch > 0 ? &p->psrc : nullptr);
   if (rc) return rc;
   p->launched = true;
   p->timed = true;
@@ -1387,6 +1494,8 @@ void gg_plan_destroy(gg_plan *p) {
   hipStreamSynchronize(engine().stream);
   if (p->exec) hipGraphExecDestroy(p->exec);
   if (p->graph) hipGraphDestroy(p->graph);
+  p->base.release();
+  p->patches.release();
   for (hipEvent_t ev : {p->ev_begin, p->ev_end, p->tev_begin, p->tev_end})
     if (ev) hipEventDestroy(ev);
   p->D.release();

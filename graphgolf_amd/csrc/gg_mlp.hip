@@ -1,0 +1,438 @@
+// gg_mlp.hip -- the swap-ranking surrogate: a small MLP trained and run on
+// the GPU with the project's own BF16 MFMA kernels (FP32 accumulate).
+//
+// Network: dims[0] -> dims[1] -> ... -> dims[L] with ReLU between layers and
+// a linear output (the predicted change of the distance sum for a swap).
+// Master parameters are FP32; every GEMM reads BF16 copies of its operands
+// and accumulates in FP32 with v_mfma_f32_32x32x8_bf16 (one 32x32 output tile
+// per wave).  Training minimises mean squared error with Adam.
+//
+// No reference counterpart: the reference proposes swaps uniformly at random
+// (general/sa.c:217-369); this network only re-orders proposals, and every
+// accepted move is still scored by the exact evaluator.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "gg_eval.h"
+#include "gg_internal.h"
+#include "gg_mlp.h"
+
+using gg::set_error;
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// FP32 -> BF16 with round-to-nearest-even (NaN kept quiet).
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return (uint16_t)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__global__ void gg_mlp_to_bf16(const float *__restrict__ in, uint16_t *__restrict__ out, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = f2bf(in[i]);
+}
+
+// C[M x N] (fp32, row-major, ldc) = op(A)[M x K] * op(B)[K x N] (+ bias[N])
+// (then ReLU when relu != 0); optionally also writes a BF16 copy (ldh).
+// op(A)(i,k) = transA ? A[k*lda + i] : A[i*lda + k]
+// op(B)(k,j) = transB ? B[j*ldb + k] : B[k*ldb + j]
+// M, N multiples of 32; K a multiple of 8.  One wave per 32x32 tile.
+//
+// Operand layout of v_mfma_f32_32x32x8_bf16: lane l holds A(l%32, 4*(l/32)+e)
+// and B(4*(l/32)+e, l%32) for e = 0..3; accumulator register r of lane l is
+// D(8*(r/4) + 4*(l/32) + r%4, l%32).
+__global__ void __launch_bounds__(64)
+gg_mlp_gemm(const uint16_t *__restrict__ A, int lda, int transA, const uint16_t *__restrict__ B,
+            int ldb, int transB, float *__restrict__ C, int ldc, uint16_t *__restrict__ Cb, int ldh,
+            const float *__restrict__ bias, int relu, int M, int N, int K) {
+  const int l = threadIdx.x;
+  const int m0 = blockIdx.x * 32, n0 = blockIdx.y * 32;
+  const int row = m0 + (l & 31), col = n0 + (l & 31);
+  const int kq = (l >> 5) * 4;
+  f32x16 acc;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+  for (int k0 = 0; k0 < K; k0 += 8) {
+    bf16x4 a, b;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int k = k0 + kq + e;
+      a[e] = (short)(transA ? A[(size_t)k * lda + row] : A[(size_t)row * lda + k]);
+      b[e] = (short)(transB ? B[(size_t)col * ldb + k] : B[(size_t)k * ldb + col]);
+    }
+    acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+  }
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int i = m0 + 8 * (r >> 2) + 4 * (l >> 5) + (r & 3);
+    const int j = n0 + (l & 31);
+    float v = acc[r];
+    if (bias) v += bias[j];
+    if (relu) v = fmaxf(v, 0.0f);
+    C[(size_t)i * ldc + j] = v;
+    if (Cb) Cb[(size_t)i * ldh + j] = f2bf(v);
+  }
+}
+
+// dy = 2 (y - t) / m for real rows (0 for padding), written into column 0 of
+// a [Mp x ldd] gradient (other columns 0); the loss sum goes to *loss.
+__global__ void gg_mlp_mse(const float *__restrict__ y, int ldy, const float *__restrict__ t, int m,
+                           int Mp, float *__restrict__ dy, int ldd, float *__restrict__ loss) {
+  __shared__ float part[256];
+  float s = 0.0f;
+  for (int i = threadIdx.x; i < Mp; i += 256) {
+    float g = 0.0f;
+    if (i < m) {
+      const float e = y[(size_t)i * ldy] - t[i];
+      s += e * e;
+      g = 2.0f * e / (float)m;
+    }
+    for (int j = 0; j < ldd; ++j) dy[(size_t)i * ldd + j] = (j == 0) ? g : 0.0f;
+  }
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *loss = part[0] / (float)m;
+}
+
+// dZ = dH * (Z > 0), and its BF16 copy.
+__global__ void gg_mlp_relu_back(const float *__restrict__ dH, const float *__restrict__ Z,
+                                 float *__restrict__ dZ, uint16_t *__restrict__ dZb, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const float g = Z[i] > 0.0f ? dH[i] : 0.0f;
+    dZ[i] = g;
+    dZb[i] = f2bf(g);
+  }
+}
+
+// db[j] = sum_i dZ[i][j] (one thread per column; M is small).
+__global__ void gg_mlp_colsum(const float *__restrict__ dZ, int M, int N, float *__restrict__ db) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  float s = 0.0f;
+  for (int i = 0; i < M; ++i) s += dZ[(size_t)i * N + j];
+  db[j] = s;
+}
+
+__global__ void gg_mlp_adam(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m1,
+                            float *__restrict__ m2, int n, float lr, float b1, float b2, float eps,
+                            float c1, float c2) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float a = b1 * m1[i] + (1.0f - b1) * gi;
+  const float b = b2 * m2[i] + (1.0f - b2) * gi * gi;
+  m1[i] = a;
+  m2[i] = b;
+  p[i] -= lr * (a / c1) / (sqrtf(b / c2) + eps);
+}
+
+// ---------------------------------------------------------------------------
+
+namespace {
+
+int pad32(int x) { return (x + 31) & ~31; }
+
+struct Layer {
+  int in, out, out_p;       // out_p: output width padded to 32
+  size_t w_off, b_off;      // offsets into the flat parameter arrays
+};
+
+}  // namespace
+
+struct gg_mlp {
+  std::vector<Layer> layers;
+  int in_dim = 0, n_params = 0, cap_m = 0;
+  int step = 0;
+  hipStream_t stream = nullptr;
+  // flat parameter buffers (fp32 master, grads, Adam moments, bf16 weights)
+  float *p = nullptr, *g = nullptr, *m1 = nullptr, *m2 = nullptr;
+  uint16_t *pb = nullptr;
+  // activations for a batch of cap_m rows
+  uint16_t *x_b = nullptr;           // [Mp][in]
+  std::vector<float *> Z;            // [Mp][out_p] pre-activation per layer
+  std::vector<uint16_t *> Hb;        // [Mp][out_p] bf16 post-activation per layer
+  float *dA = nullptr, *dB = nullptr;  // [Mp][maxw] gradient scratch
+  uint16_t *dAb = nullptr;
+  float *x_f = nullptr, *t_f = nullptr, *loss = nullptr;
+  int maxw = 0;
+};
+
+#define MLP_HIP(call)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return set_error(GG_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_));     \
+  } while (0)
+
+static int mlp_reserve(gg_mlp *M, int m) {
+  const int Mp = pad32(m);
+  if (Mp <= M->cap_m) return GG_OK;
+  for (float *z : M->Z) hipFree(z);
+  for (uint16_t *h : M->Hb) hipFree(h);
+  M->Z.clear();
+  M->Hb.clear();
+  hipFree(M->x_b);
+  hipFree(M->dA);
+  hipFree(M->dB);
+  hipFree(M->dAb);
+  hipFree(M->x_f);
+  hipFree(M->t_f);
+  M->cap_m = 0;
+  MLP_HIP(hipMalloc(&M->x_b, (size_t)Mp * M->in_dim * 2));
+  MLP_HIP(hipMalloc(&M->x_f, (size_t)Mp * M->in_dim * 4));
+  MLP_HIP(hipMalloc(&M->t_f, (size_t)Mp * 4));
+  for (const Layer &L : M->layers) {
+    float *z;
+    uint16_t *h;
+    MLP_HIP(hipMalloc(&z, (size_t)Mp * L.out_p * 4));
+    MLP_HIP(hipMalloc(&h, (size_t)Mp * L.out_p * 2));
+    M->Z.push_back(z);
+    M->Hb.push_back(h);
+  }
+  MLP_HIP(hipMalloc(&M->dA, (size_t)Mp * M->maxw * 4));
+  MLP_HIP(hipMalloc(&M->dB, (size_t)Mp * M->maxw * 4));
+  MLP_HIP(hipMalloc(&M->dAb, (size_t)Mp * M->maxw * 2));
+  M->cap_m = Mp;
+  return GG_OK;
+}
+
+static void launch_gemm(hipStream_t s, const uint16_t *A, int lda, int tA, const uint16_t *B, int ldb,
+                        int tB, float *C, int ldc, uint16_t *Cb, int ldh, const float *bias, int relu,
+                        int Mr, int Nr, int K) {
+  hipLaunchKernelGGL(gg_mlp_gemm, dim3(Mr / 32, Nr / 32), dim3(64), 0, s, A, lda, tA, B, ldb, tB, C,
+                     ldc, Cb, ldh, bias, relu, Mr, Nr, K);
+}
+
+// Uploads x (m rows, fp32) and runs the forward pass; output in Z.back() col 0.
+static int mlp_forward(gg_mlp *M, const float *x_host, int m, bool device_x) {
+  int rc = mlp_reserve(M, m);
+  if (rc) return rc;
+  const int Mp = pad32(m), in = M->in_dim;
+  MLP_HIP(hipMemsetAsync(M->x_f, 0, (size_t)Mp * in * 4, M->stream));
+  MLP_HIP(hipMemcpyAsync(M->x_f, x_host, (size_t)m * in * 4,
+                         device_x ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, M->stream));
+  const int nx = Mp * in;
+  hipLaunchKernelGGL(gg_mlp_to_bf16, dim3((nx + 255) / 256), dim3(256), 0, M->stream, M->x_f, M->x_b, nx);
+  MLP_HIP(hipGetLastError());
+  const uint16_t *h = M->x_b;
+  int hw = in;
+  for (size_t li = 0; li < M->layers.size(); ++li) {
+    const Layer &L = M->layers[li];
+    const bool last = li + 1 == M->layers.size();
+    launch_gemm(M->stream, h, hw, 0, M->pb + L.w_off, L.out_p, 0, M->Z[li], L.out_p, M->Hb[li],
+                L.out_p, M->p + L.b_off, last ? 0 : 1, Mp, L.out_p, L.in);
+    h = M->Hb[li];
+    hw = L.out_p;
+  }
+  MLP_HIP(hipGetLastError());
+  return GG_OK;
+}
+
+// Backward pass from targets t (device t_f); fills M->g.
+static int mlp_backward(gg_mlp *M, int m) {
+  const int Mp = pad32(m);
+  const int nl = (int)M->layers.size();
+  MLP_HIP(hipMemsetAsync(M->g, 0, (size_t)M->n_params * 4, M->stream));
+  const Layer &Lo = M->layers[nl - 1];
+  // dZ_last into dA (fp32) + bf16 copy
+  hipLaunchKernelGGL(gg_mlp_mse, dim3(1), dim3(256), 0, M->stream, M->Z[nl - 1], Lo.out_p, M->t_f, m,
+                     Mp, M->dA, Lo.out_p, M->loss);
+  int n = Mp * Lo.out_p;
+  hipLaunchKernelGGL(gg_mlp_to_bf16, dim3((n + 255) / 256), dim3(256), 0, M->stream, M->dA, M->dAb, n);
+  for (int li = nl - 1; li >= 0; --li) {
+    const Layer &L = M->layers[li];
+    const uint16_t *hin = li ? M->Hb[li - 1] : M->x_b;
+    const int win = li ? M->layers[li - 1].out_p : M->in_dim;
+    // dW = H_in^T * dZ   ([win x out_p]; only the first L.in rows are real)
+    launch_gemm(M->stream, hin, win, 1, M->dAb, L.out_p, 0, M->g + L.w_off, L.out_p, nullptr, 0,
+                nullptr, 0, pad32(L.in), L.out_p, Mp);
+    hipLaunchKernelGGL(gg_mlp_colsum, dim3((L.out_p + 63) / 64), dim3(64), 0, M->stream, M->dA, Mp,
+                       L.out_p, M->g + L.b_off);
+    if (li == 0) break;
+    // dH_in = dZ * W^T   ([Mp x win])
+    launch_gemm(M->stream, M->dAb, L.out_p, 0, M->pb + L.w_off, L.out_p, 1, M->dB, win, nullptr, 0,
+                nullptr, 0, Mp, win, L.out_p);
+    n = Mp * win;
+    hipLaunchKernelGGL(gg_mlp_relu_back, dim3((n + 255) / 256), dim3(256), 0, M->stream, M->dB,
+                       M->Z[li - 1], M->dA, M->dAb, n);
+  }
+  MLP_HIP(hipGetLastError());
+  return GG_OK;
+}
+
+static int refresh_bf16(gg_mlp *M) {
+  hipLaunchKernelGGL(gg_mlp_to_bf16, dim3((M->n_params + 255) / 256), dim3(256), 0, M->stream, M->p,
+                     M->pb, M->n_params);
+  MLP_HIP(hipGetLastError());
+  return GG_OK;
+}
+
+extern "C" {
+
+gg_mlp *gg_mlp_create(int n_layers, const int *dims, uint64_t seed) {
+  if (n_layers < 1 || !dims) {
+    set_error(GG_ERR_INVALID, "bad MLP shape");
+    return nullptr;
+  }
+  for (int i = 0; i <= n_layers; ++i)
+    if (dims[i] < 1) {
+      set_error(GG_ERR_INVALID, "layer widths must be positive");
+      return nullptr;
+    }
+  if (dims[0] % 32) {
+    set_error(GG_ERR_INVALID, "the input width must be a multiple of 32");
+    return nullptr;
+  }
+  if (gg_init(-1) != GG_OK) return nullptr;
+  gg_mlp *M = new gg_mlp();
+  M->in_dim = dims[0];
+  size_t off = 0;
+  int prev_p = dims[0];
+  M->maxw = dims[0];
+  for (int i = 0; i < n_layers; ++i) {
+    Layer L;
+    L.in = prev_p;  // the previous layer's padded width (padding columns stay zero)
+    L.out = dims[i + 1];
+    L.out_p = pad32(dims[i + 1]);
+    L.w_off = off;
+    off += (size_t)L.in * L.out_p;
+    L.b_off = off;
+    off += L.out_p;
+    M->layers.push_back(L);
+    M->maxw = std::max(M->maxw, L.out_p);
+    prev_p = L.out_p;
+  }
+  M->n_params = (int)off;
+  std::vector<float> host(off, 0.0f);
+  // He-uniform initialisation from a splitmix64 stream; padded rows/columns zero.
+  uint64_t st = seed;
+  auto rnd = [&]() {
+    st += 0x9E3779B97F4A7C15ull;
+    uint64_t z = st;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) / 9007199254740992.0;
+  };
+  int real_in = dims[0];
+  for (int i = 0; i < n_layers; ++i) {
+    const Layer &L = M->layers[i];
+    const double lim = std::sqrt(6.0 / real_in);
+    for (int r = 0; r < real_in; ++r)
+      for (int c = 0; c < L.out; ++c) host[L.w_off + (size_t)r * L.out_p + c] = (float)((2 * rnd() - 1) * lim);
+    real_in = L.out;
+  }
+  hipStreamCreateWithFlags(&M->stream, hipStreamNonBlocking);
+  bool ok = hipMalloc(&M->p, off * 4) == hipSuccess && hipMalloc(&M->g, off * 4) == hipSuccess &&
+            hipMalloc(&M->m1, off * 4) == hipSuccess && hipMalloc(&M->m2, off * 4) == hipSuccess &&
+            hipMalloc(&M->pb, off * 2) == hipSuccess && hipMalloc(&M->loss, 4) == hipSuccess;
+  if (!ok) {
+    set_error(GG_ERR_HIP, "MLP allocation failed");
+    gg_mlp_destroy(M);
+    return nullptr;
+  }
+  hipMemcpy(M->p, host.data(), off * 4, hipMemcpyHostToDevice);
+  hipMemset(M->m1, 0, off * 4);
+  hipMemset(M->m2, 0, off * 4);
+  hipMemset(M->g, 0, off * 4);
+  if (refresh_bf16(M) || hipStreamSynchronize(M->stream) != hipSuccess) {
+    gg_mlp_destroy(M);
+    return nullptr;
+  }
+  return M;
+}
+
+void gg_mlp_destroy(gg_mlp *M) {
+  if (!M) return;
+  if (M->stream) hipStreamSynchronize(M->stream);
+  for (float *z : M->Z) hipFree(z);
+  for (uint16_t *h : M->Hb) hipFree(h);
+  for (void *q : {(void *)M->p, (void *)M->g, (void *)M->m1, (void *)M->m2, (void *)M->pb,
+                  (void *)M->x_b, (void *)M->dA, (void *)M->dB, (void *)M->dAb, (void *)M->x_f,
+                  (void *)M->t_f, (void *)M->loss})
+    if (q) hipFree(q);
+  if (M->stream) hipStreamDestroy(M->stream);
+  delete M;
+}
+
+int gg_mlp_param_count(const gg_mlp *M) { return M ? M->n_params : 0; }
+
+int gg_mlp_layout(const gg_mlp *M, int *rows, int *cols_padded, int *w_off, int *b_off) {
+  if (!M) return set_error(GG_ERR_INVALID, "null MLP");
+  for (size_t i = 0; i < M->layers.size(); ++i) {
+    rows[i] = M->layers[i].in;
+    cols_padded[i] = M->layers[i].out_p;
+    w_off[i] = (int)M->layers[i].w_off;
+    b_off[i] = (int)M->layers[i].b_off;
+  }
+  return (int)M->layers.size();
+}
+
+int gg_mlp_get_params(gg_mlp *M, float *out) {
+  if (!M || !out) return set_error(GG_ERR_INVALID, "bad arguments");
+  MLP_HIP(hipMemcpyAsync(out, M->p, (size_t)M->n_params * 4, hipMemcpyDeviceToHost, M->stream));
+  MLP_HIP(hipStreamSynchronize(M->stream));
+  return GG_OK;
+}
+
+int gg_mlp_set_params(gg_mlp *M, const float *in) {
+  if (!M || !in) return set_error(GG_ERR_INVALID, "bad arguments");
+  MLP_HIP(hipMemcpyAsync(M->p, in, (size_t)M->n_params * 4, hipMemcpyHostToDevice, M->stream));
+  int rc = refresh_bf16(M);
+  if (rc) return rc;
+  MLP_HIP(hipStreamSynchronize(M->stream));
+  return GG_OK;
+}
+
+int gg_mlp_forward(gg_mlp *M, const float *x, int m, float *y) {
+  if (!M || !x || !y || m < 1) return set_error(GG_ERR_INVALID, "bad arguments");
+  int rc = mlp_forward(M, x, m, false);
+  if (rc) return rc;
+  const Layer &Lo = M->layers.back();
+  MLP_HIP(hipMemcpy2DAsync(y, sizeof(float), M->Z.back(), (size_t)Lo.out_p * 4, sizeof(float), m,
+                           hipMemcpyDeviceToHost, M->stream));
+  MLP_HIP(hipStreamSynchronize(M->stream));
+  return GG_OK;
+}
+
+int gg_mlp_gradient(gg_mlp *M, const float *x, const float *t, int m, float *loss, float *grad) {
+  if (!M || !x || !t || m < 1) return set_error(GG_ERR_INVALID, "bad arguments");
+  int rc = mlp_forward(M, x, m, false);
+  if (rc) return rc;
+  MLP_HIP(hipMemcpyAsync(M->t_f, t, (size_t)m * 4, hipMemcpyHostToDevice, M->stream));
+  if ((rc = mlp_backward(M, m))) return rc;
+  if (grad) MLP_HIP(hipMemcpyAsync(grad, M->g, (size_t)M->n_params * 4, hipMemcpyDeviceToHost, M->stream));
+  if (loss) MLP_HIP(hipMemcpyAsync(loss, M->loss, 4, hipMemcpyDeviceToHost, M->stream));
+  MLP_HIP(hipStreamSynchronize(M->stream));
+  return GG_OK;
+}
+
+int gg_mlp_train_step(gg_mlp *M, const float *x, const float *t, int m, float lr, float *loss) {
+  int rc = gg_mlp_gradient(M, x, t, m, loss, nullptr);
+  if (rc) return rc;
+  ++M->step;
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float c1 = 1.0f - std::pow(b1, (float)M->step), c2 = 1.0f - std::pow(b2, (float)M->step);
+  hipLaunchKernelGGL(gg_mlp_adam, dim3((M->n_params + 255) / 256), dim3(256), 0, M->stream, M->p, M->g,
+                     M->m1, M->m2, M->n_params, lr, b1, b2, 1e-8f, c1, c2);
+  MLP_HIP(hipGetLastError());
+  if ((rc = refresh_bf16(M))) return rc;
+  MLP_HIP(hipStreamSynchronize(M->stream));
+  return GG_OK;
+}
+
+}  // extern "C"

@@ -63,3 +63,85 @@ def sa_general(edges, nodes: int, degree: int, seed: int, ncalcs: int, max_temp:
     return SaResult(e, st.steps, st.accepts, st.evaluations, st.best_diameter, st.best_aspl,
                     st.seconds_total, st.seconds_eval,
                     ld[:n_log] if log else None, ls[:n_log] if log else None)
+
+
+class BsaParams(ctypes.Structure):
+    _fields_ = [("max_steps", ctypes.c_int64), ("time_budget_s", ctypes.c_double),
+                ("target_aspl", ctypes.c_double), ("proposals", ctypes.c_int32),
+                ("exact", ctypes.c_int32), ("explore", ctypes.c_int32),
+                ("use_surrogate", ctypes.c_int32), ("max_temp", ctypes.c_double),
+                ("min_temp", ctypes.c_double), ("seed", ctypes.c_uint64),
+                ("train_steps", ctypes.c_int32), ("batch_train", ctypes.c_int32),
+                ("lr", ctypes.c_float), ("hist_maxd", ctypes.c_int32),
+                ("feature_refresh", ctypes.c_int32), ("log_capacity", ctypes.c_int32)]
+
+
+class BsaStats(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_int64), ("accepts", ctypes.c_int64),
+                ("evaluations", ctypes.c_int64), ("best_diameter", ctypes.c_int32),
+                ("pad", ctypes.c_int32), ("best_aspl", ctypes.c_double),
+                ("seconds_total", ctypes.c_double), ("seconds_exact", ctypes.c_double),
+                ("seconds_surrogate", ctypes.c_double), ("seconds_features", ctypes.c_double),
+                ("seconds_to_target", ctypes.c_double), ("surrogate_rank_corr", ctypes.c_double)]
+
+
+@dataclass
+class BatchedResult:
+    edges: np.ndarray
+    stats: dict
+    log_time: np.ndarray
+    log_best_aspl: np.ndarray
+    log_best_diam: np.ndarray
+    accepted: np.ndarray        # [k, 3] (line0, line1, variant)
+    accepted_diam: np.ndarray
+    accepted_sum: np.ndarray
+
+
+def sa_batched(edges, nodes: int, degree: int, *, max_steps: int = 100, time_budget_s: float = 0.0,
+               target_aspl: float = 0.0, proposals: int = 4096, exact: int = 256,
+               explore: int = 16, use_surrogate: bool = True, max_temp: float = 100.0,
+               min_temp: float = 0.217147, seed: int = 1, train_steps: int = 4,
+               batch_train: int = 512, lr: float = 1e-3, hist_maxd: int = 8,
+               feature_refresh: int = 1, log_capacity: int = 100000,
+               accepted_capacity: int = 100000) -> BatchedResult:
+    """Batched (and, with use_surrogate, MLP-ranked) SA on the GPU; see
+    include/gg_host.h.  exact = proposals = 1 without the surrogate is plain
+    single-graph GPU SA with a device-resident adjacency."""
+    lib = L.lib()
+    if not getattr(lib, "_bsa_sigs", False):
+        lib.gg_sa_batched.argtypes = [ctypes.c_int, ctypes.c_int, L.i32p,
+                                      ctypes.POINTER(BsaParams), ctypes.POINTER(BsaStats),
+                                      ctypes.POINTER(ctypes.c_double),
+                                      ctypes.POINTER(ctypes.c_double), L.i32p, L.i32p, L.i32p,
+                                      L.u64p, ctypes.c_int]
+        lib.gg_sa_batched.restype = ctypes.c_int
+        lib.gg_apply_move.argtypes = [ctypes.c_int, L.i32p, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_int]
+        lib.gg_apply_move.restype = ctypes.c_int
+        lib._bsa_sigs = True
+    e = L.c32(edges).reshape(-1, 2).copy()
+    p = BsaParams(max_steps, time_budget_s, target_aspl, proposals, exact, explore,
+                  int(use_surrogate), max_temp, min_temp, seed, train_steps, batch_train, lr,
+                  hist_maxd, feature_refresh, log_capacity)
+    st = BsaStats()
+    lt = np.zeros(max(log_capacity, 1))
+    la = np.zeros(max(log_capacity, 1))
+    ld = np.zeros(max(log_capacity, 1), dtype=np.int32)
+    acc = np.zeros((max(accepted_capacity, 1), 3), dtype=np.int32)
+    ad = np.zeros(max(accepted_capacity, 1), dtype=np.int32)
+    asum = np.zeros(max(accepted_capacity, 1), dtype=np.uint64)
+    dp = ctypes.POINTER(ctypes.c_double)
+    rc = lib.gg_sa_batched(nodes, degree, L.ptr(e), ctypes.byref(p), ctypes.byref(st),
+                           lt.ctypes.data_as(dp), la.ctypes.data_as(dp), L.ptr(ld), L.ptr(acc),
+                           L.ptr(ad), asum.ctypes.data_as(L.u64p), accepted_capacity)
+    L.check(rc)
+    stats = {k: getattr(st, k) for k, _ in BsaStats._fields_ if k != "pad"}
+    ns = min(int(st.steps), log_capacity)
+    na = min(int(st.accepts), accepted_capacity)
+    return BatchedResult(e, stats, lt[:ns], la[:ns], ld[:ns], acc[:na], ad[:na], asum[:na])
+
+
+def apply_move(edges: np.ndarray, line0: int, line1: int, variant: int) -> None:
+    """Applies a logged batched-SA move to `edges` in place."""
+    L.check(L.lib().gg_apply_move(edges.shape[0], L.ptr(edges), int(line0), int(line1),
+                                  int(variant)))

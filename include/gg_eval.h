@@ -98,6 +98,17 @@ int gg_plan_run(gg_plan *p, gg_result *out);        /* replay + wait + results *
 int gg_plan_launch(gg_plan *p);                     /* replay only (async) */
 int gg_plan_wait(gg_plan *p, gg_result *out);       /* wait for the last launch */
 void gg_plan_destroy(gg_plan *p);
+/* Patched plans: `count` proposals that all share one base graph (shape
+ * `*shape`, which must carry GG_GRAPH_SYMMETRIC) and differ from it in at most
+ * `max_patch` adjacency entries each.  The base table lives on the device
+ * (gg_plan_set_base / gg_plan_base_device); patches are int32
+ * [count][max_patch][2] pairs (v*degree + j, new neighbour), index -1 = unused.
+ * The prep kernel applies them while converting, so proposals cost no
+ * per-proposal adjacency transfer. */
+gg_plan *gg_plan_create_patched(int count, const gg_graph *shape, int max_patch);
+int gg_plan_set_base(gg_plan *p, const int32_t *adj_host);
+int32_t *gg_plan_base_device(gg_plan *p);
+int gg_plan_set_patches(gg_plan *p, const int32_t *patches_host);
 /* Enqueues the plan's kernels directly (not through the graph) with HIP
  * events around the tile-BFS launches; complete it with gg_plan_wait(). */
 int gg_plan_launch_timed(gg_plan *p);

@@ -78,6 +78,60 @@ int gg_sa_general(int nodes, int degree, int32_t *edges, const gg_sa_params *par
                   gg_eval_fn eval, void *eval_ctx, gg_sa_stats *stats, int32_t *log_diam,
                   uint64_t *log_sum);
 
+/* ---- Batched, surrogate-ranked simulated annealing (no reference
+ * counterpart; the accept rule is the reference's, general/sa.c:371-411).
+ *
+ * Each step draws `proposals` random 2-opt moves of the current graph; with
+ * use_surrogate the MLP (include/gg_mlp.h) ranks them from per-vertex
+ * distance-histogram features and the best `exact - explore` plus `explore`
+ * random ones are scored exactly in one patched plan launch (without it,
+ * `exact` random moves are scored).  The best scored move then faces the
+ * reference's accept rule: smaller diameter wins; equal diameter accepts a
+ * non-increasing S, else Metropolis exp(-dS/T).  Every exact (features, dS)
+ * pair trains the MLP on the device.  exact = proposals = 1 without the
+ * surrogate is plain single-graph GPU SA on a device-resident adjacency.
+ * groups = 1, no added centres. */
+typedef struct gg_bsa_params {
+  int64_t max_steps;
+  double time_budget_s;   /* stop after this wall time (0: none) */
+  double target_aspl;     /* stop once best ASPL <= target (0: none) */
+  int32_t proposals;
+  int32_t exact;
+  int32_t explore;
+  int32_t use_surrogate;
+  double max_temp, min_temp; /* geometric schedule over max_steps */
+  uint64_t seed;
+  int32_t train_steps;    /* Adam steps per SA step */
+  int32_t batch_train;    /* rows per Adam step */
+  float lr;
+  int32_t hist_maxd;      /* distance histogram depth for features (<= 15) */
+  int32_t feature_refresh;/* recompute features every k accepted moves */
+  int32_t log_capacity;   /* entries in the log arrays */
+} gg_bsa_params;
+
+typedef struct gg_bsa_stats {
+  int64_t steps, accepts, evaluations;
+  int32_t best_diameter, pad;
+  double best_aspl;
+  double seconds_total, seconds_exact, seconds_surrogate, seconds_features;
+  double seconds_to_target; /* -1 when the target was not reached */
+  double surrogate_rank_corr; /* Spearman correlation, predicted vs exact, last 64 steps */
+} gg_bsa_stats;
+
+/* log_time[k], log_best_aspl[k], log_best_diam[k]: after step k.
+ * accepted[k*3 + {0,1,2}] = (line0, line1, variant) of the k-th accepted move,
+ * accepted_diam[k] / accepted_sum[k] its exact diameter and S, so the run can
+ * be replayed from the initial edges and every accepted graph re-checked.
+ * On return `edges` holds the best graph found. */
+int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa_params *params,
+                  gg_bsa_stats *stats, double *log_time, double *log_best_aspl,
+                  int32_t *log_best_diam, int32_t *accepted, int32_t *accepted_diam,
+                  uint64_t *accepted_sum, int accepted_capacity);
+
+/* Applies one logged move (line0, line1, variant) to an edge list the way
+ * gg_sa_batched does (for replays). */
+int gg_apply_move(int lines, int32_t *edges, int line0, int line1, int variant);
+
 #ifdef __cplusplus
 }
 #endif
