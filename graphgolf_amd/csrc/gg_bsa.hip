@@ -1,0 +1,722 @@
+// gg_bsa.hip -- batched, surrogate-ranked simulated annealing over the
+// patched-plan evaluator (gg_sa_batched, include/gg_host.h).
+//
+// No reference counterpart as a whole: the reference draws one 2-opt move at
+// a time (general/sa.c:217-369) and scores it with evaluation().  Here every
+// step scores `exact` candidate graphs (the current graph plus one or more
+// vertex-disjoint moves) in one launch of a patched plan; a device-side
+// feature kernel and the BF16 MLP (gg_mlp.hip) decide which moves go into
+// the candidates.  The accept rule is the reference's (general/sa.c:371-411)
+// applied to the best candidate, so every accepted graph is scored exactly.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "gg_device.h"
+#include "gg_eval.h"
+#include "gg_host.h"
+#include "gg_internal.h"
+#include "gg_mlp.h"
+#include "gg_sa_common.h"
+
+using gg::set_error;
+
+namespace {
+constexpr int kFeat = 64;        // MLP input width
+constexpr int kVStat = 10;       // per-vertex statistics (see VertexStats)
+constexpr int kMaxCombine = 16;  // moves per candidate (4 patch entries each)
+constexpr int kBlock = 256;
+}  // namespace
+
+__device__ __forceinline__ int bsa_block_sum(int v, int *red) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  int t = 0;
+  for (int w = 0; w < kBlock / 64; ++w) t += red[w];
+  __syncthreads();
+  return t;
+}
+
+// Features of proposal p = (a,b),(c,d) -> variant r, one workgroup each.
+// Removed pairs (a,b), (c,d); added pairs (p1,q1), (p2,q2) with
+// r = 0: (a,d), (c,b) and r = 1: (a,c), (b,d).  Bx / B2x are the closed 1- and
+// 2-hop balls of x in the current graph `adj` (row-major n x d).
+//   f[10e + k]   vstat[endpoint e][k]            e over a, b, c, d
+//   f[40 + 4q]   |B1x & B1y| / d                 q over the four pairs
+//   f[41 + 4q]   |B1x & B2y| / d
+//   f[42 + 4q]   |B2x & B2y| / d^2
+//   f[43 + 4q]   (vstat[x][7] + vstat[y][7]) / 2 (mean relative distance sum)
+//   f[56]        r;  the rest 0
+// LDS: two n-bit bitmaps (B2x, B2y), reused for the four pairs.
+__global__ void __launch_bounds__(kBlock)
+gg_bsa_features(const int32_t *__restrict__ adj, int n, int d, const int32_t *__restrict__ props,
+                const float *__restrict__ vstat, float *__restrict__ feats) {
+  extern __shared__ uint32_t bm[];
+  __shared__ int red[kBlock / 64];
+  __shared__ int cnt[4][3];
+  const int words = (n + 31) >> 5;
+  const int tid = threadIdx.x;
+  const int32_t *pr = props + 5 * (size_t)blockIdx.x;
+  const int a = pr[0], b = pr[1], c = pr[2], e = pr[3], r = pr[4];
+  const int ball = 1 + d + d * d;
+  for (int q = 0; q < 4; ++q) {
+    const int x = q == 0 ? a : q == 1 ? c : q == 2 ? a : (r ? b : c);
+    const int y = q == 0 ? b : q == 1 ? e : q == 2 ? (r ? c : e) : (r ? e : b);
+    for (int i = tid; i < 2 * words; i += kBlock) bm[i] = 0;
+    __syncthreads();
+    for (int i = tid; i < 2 * ball; i += kBlock) {
+      const int side = i >= ball;
+      const int k = i - side * ball;
+      const int root = side ? y : x;
+      int v;
+      if (k == 0) {
+        v = root;
+      } else if (k <= d) {
+        v = adj[(size_t)root * d + k - 1];
+      } else {
+        const int j = (k - 1 - d) / d, l = (k - 1 - d) - j * d;
+        v = adj[(size_t)adj[(size_t)root * d + j] * d + l];
+      }
+      atomicOr(&bm[side * words + (v >> 5)], 1u << (v & 31));
+    }
+    __syncthreads();
+    int c0 = 0, c1 = 0, c2 = 0;
+    for (int k = tid; k <= d; k += kBlock) {
+      const int u = k == 0 ? x : adj[(size_t)x * d + k - 1];
+      c1 += (bm[words + (u >> 5)] >> (u & 31)) & 1;
+      int in1 = u == y;
+      for (int j = 0; j < d; ++j) in1 |= adj[(size_t)y * d + j] == u;
+      c0 += in1;
+    }
+    for (int w = tid; w < words; w += kBlock) c2 += __popc(bm[w] & bm[words + w]);
+    c0 = bsa_block_sum(c0, red);
+    c1 = bsa_block_sum(c1, red);
+    c2 = bsa_block_sum(c2, red);
+    if (tid == 0) {
+      cnt[q][0] = c0;
+      cnt[q][1] = c1;
+      cnt[q][2] = c2;
+    }
+  }
+  __syncthreads();
+  float *f = feats + (size_t)blockIdx.x * kFeat;
+  for (int i = tid; i < kFeat; i += kBlock) {
+    float v = 0.0f;
+    if (i < 40) {
+      const int end = i / 10;
+      const int vx = end == 0 ? a : end == 1 ? b : end == 2 ? c : e;
+      v = vstat[(size_t)vx * kVStat + (i - 10 * end)];
+    } else if (i < 56) {
+      const int q = (i - 40) >> 2, k = (i - 40) & 3;
+      const int x = q == 0 ? a : q == 1 ? c : q == 2 ? a : (r ? b : c);
+      const int y = q == 0 ? b : q == 1 ? e : q == 2 ? (r ? c : e) : (r ? e : b);
+      if (k == 0) v = (float)cnt[q][0] / d;
+      else if (k == 1) v = (float)cnt[q][1] / d;
+      else if (k == 2) v = (float)cnt[q][2] / ((float)d * d);
+      else v = 0.5f * (vstat[(size_t)x * kVStat + 7] + vstat[(size_t)y * kVStat + 7]);
+    } else if (i == 56) {
+      v = (float)r;
+    }
+    f[i] = v;
+  }
+}
+
+// dst[dst_row[i]] = src[src_row[i]] for i < rows (rows of kFeat floats; a
+// null index array means the identity).
+__global__ void gg_bsa_copy_rows(const float *__restrict__ src, const int32_t *__restrict__ src_row,
+                                 float *__restrict__ dst, const int32_t *__restrict__ dst_row,
+                                 int rows) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows * kFeat) return;
+  const int row = i / kFeat, k = i - row * kFeat;
+  const int s = src_row ? src_row[row] : row, t = dst_row ? dst_row[row] : row;
+  dst[(size_t)t * kFeat + k] = src[(size_t)s * kFeat + k];
+}
+
+namespace {
+
+using gg::EdgeSet;
+using gg::ekey;
+using gg::now_s;
+using gg::Rng;
+
+#define BSA_HIP(call)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      return set_error(GG_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+// A 2-opt move on edge-list lines t0 = (a,b), t1 = (c,d):
+// r = 0 -> (a,d), (c,b);  r = 1 -> (a,c), (b,d).
+struct Prop {
+  int t0, t1, r, a, b, c, d;
+};
+
+struct Slots {
+  int at[4], val[4];
+};
+
+struct SaGraph {
+  int n, d, lines;
+  int32_t *edges;
+  std::vector<int32_t> adj;  // current graph, row-major n x d
+  EdgeSet set;
+  SaGraph(int n_, int d_, int32_t *e)
+      : n(n_), d(d_), lines(n_ * d_ / 2), edges(e), adj((size_t)n_ * d_), set((size_t)n_ * d_ / 2) {}
+  int init() {
+    int rc = gg_edges_to_adj(n, lines, d, edges, adj.data());
+    if (rc) return rc;
+    for (int i = 0; i < lines; ++i) {
+      const int u = edges[2 * i], v = edges[2 * i + 1];
+      if (u == v || set.has(ekey(u, v)))
+        return set_error(GG_ERR_INVALID, "batched SA needs a simple graph (line %d: %d-%d)", i, u, v);
+      set.add(ekey(u, v));
+    }
+    for (int v = 0; v < n; ++v)
+      for (int j = 0; j < d; ++j)
+        if (adj[(size_t)v * d + j] == v)
+          return set_error(GG_ERR_INVALID, "batched SA needs a %d-regular graph", d);
+    return GG_OK;
+  }
+  // Reads the endpoints; false when the two lines share a vertex.
+  bool load(Prop &p) const {
+    p.a = edges[2 * p.t0];
+    p.b = edges[2 * p.t0 + 1];
+    p.c = edges[2 * p.t1];
+    p.d = edges[2 * p.t1 + 1];
+    return p.t0 != p.t1 && p.a != p.c && p.a != p.d && p.b != p.c && p.b != p.d;
+  }
+  void draw(Rng &rng, Prop &p) const {
+    do {
+      p.t0 = (int)rng.below((uint32_t)lines);
+      p.t1 = (int)rng.below((uint32_t)lines);
+      p.r = (int)rng.below(2);
+    } while (!load(p));
+  }
+  // True when neither added edge exists yet (the move keeps the graph simple).
+  bool fresh(const Prop &p) const {
+    const int q1 = p.r ? p.c : p.d, p2 = p.r ? p.b : p.c, q2 = p.r ? p.d : p.b;
+    return !set.has(ekey(p.a, q1)) && !set.has(ekey(p2, q2));
+  }
+  int find(int y, int v) const {
+    for (int x = 0; x < d; ++x)
+      if (adj[(size_t)y * d + x] == v) return x;
+    return -1;
+  }
+  // Adjacency entries the move rewrites: a's b, b's a, c's d, d's c.
+  bool slots(const Prop &p, Slots &s) const {
+    const int x0 = find(p.a, p.b), x1 = find(p.b, p.a), x2 = find(p.c, p.d), x3 = find(p.d, p.c);
+    if (x0 < 0 || x1 < 0 || x2 < 0 || x3 < 0) return false;
+    s.at[0] = p.a * d + x0;
+    s.at[1] = p.b * d + x1;
+    s.at[2] = p.c * d + x2;
+    s.at[3] = p.d * d + x3;
+    s.val[0] = p.r ? p.c : p.d;
+    s.val[1] = p.r ? p.d : p.c;
+    s.val[2] = p.r ? p.a : p.b;
+    s.val[3] = p.r ? p.b : p.a;
+    return true;
+  }
+  void apply(const Prop &p, const Slots &s) {
+    for (int k = 0; k < 4; ++k) adj[s.at[k]] = s.val[k];
+    set.remove(ekey(p.a, p.b));
+    set.remove(ekey(p.c, p.d));
+    const int e0a = p.a, e0b = p.r ? p.c : p.d, e1a = p.r ? p.b : p.c, e1b = p.r ? p.d : p.b;
+    set.add(ekey(e0a, e0b));
+    set.add(ekey(e1a, e1b));
+    edges[2 * p.t0] = e0a;
+    edges[2 * p.t0 + 1] = e0b;
+    edges[2 * p.t1] = e1a;
+    edges[2 * p.t1 + 1] = e1b;
+  }
+};
+
+// Per-vertex statistics from distance histograms (refreshed every few
+// accepts): h_1..h_7 / n, S_v / mean(S) - 1, ecc_v / max ecc, and the mean of
+// S_v / mean(S) - 1 over v's neighbours.
+struct VertexStats {
+  int maxd = 8;
+  std::vector<int32_t> hist;
+  std::vector<float> host;  // [n][kVStat]
+  int refresh(const SaGraph &g) {
+    const int n = g.n;
+    hist.assign((size_t)n * (maxd + 1), 0);
+    int rc = gg_distance_histogram(n, g.d, g.adj.data(), maxd, hist.data());
+    if (rc) return rc;
+    std::vector<double> S(n);
+    std::vector<int> ecc(n);
+    double tot = 0;
+    int diam = 1;
+    for (int v = 0; v < n; ++v) {
+      const int32_t *h = &hist[(size_t)v * (maxd + 1)];
+      double s = 0;
+      int e = 0;
+      for (int k = 1; k <= maxd; ++k) {
+        s += (double)k * h[k];
+        if (h[k]) e = k;
+      }
+      S[v] = s;
+      ecc[v] = e;
+      tot += s;
+      diam = std::max(diam, e);
+    }
+    const double mean = std::max(tot / n, 1.0);
+    host.assign((size_t)n * kVStat, 0.0f);
+    for (int v = 0; v < n; ++v) {
+      float *o = &host[(size_t)v * kVStat];
+      const int32_t *h = &hist[(size_t)v * (maxd + 1)];
+      for (int k = 1; k <= 7; ++k) o[k - 1] = k <= maxd ? (float)h[k] / n : 0.0f;
+      o[7] = (float)(S[v] / mean - 1.0);
+      o[8] = (float)ecc[v] / diam;
+    }
+    for (int v = 0; v < n; ++v) {
+      double acc = 0;
+      for (int j = 0; j < g.d; ++j) acc += host[(size_t)g.adj[(size_t)v * g.d + j] * kVStat + 7];
+      host[(size_t)v * kVStat + 9] = (float)(acc / g.d);
+    }
+    return GG_OK;
+  }
+};
+
+double spearman(const std::vector<double> &x, const std::vector<double> &y) {
+  const size_t n = x.size();
+  if (n < 3) return 0;
+  auto ranks = [&](const std::vector<double> &v) {
+    std::vector<size_t> idx(n);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::sort(idx.begin(), idx.end(), [&](size_t i, size_t j) { return v[i] < v[j]; });
+    std::vector<double> r(n);
+    for (size_t i = 0; i < n; ++i) r[idx[i]] = (double)i;
+    return r;
+  };
+  std::vector<double> rx = ranks(x), ry = ranks(y);
+  double mx = (n - 1) / 2.0, sxy = 0, sxx = 0, syy = 0;
+  for (size_t i = 0; i < n; ++i) {
+    sxy += (rx[i] - mx) * (ry[i] - mx);
+    sxx += (rx[i] - mx) * (rx[i] - mx);
+    syy += (ry[i] - mx) * (ry[i] - mx);
+  }
+  return (sxx > 0 && syy > 0) ? sxy / std::sqrt(sxx * syy) : 0;
+}
+
+// A candidate graph: the current graph plus nm vertex-disjoint moves.
+struct Cand {
+  int nm = 0;
+  int prop[kMaxCombine];
+  Slots sl[kMaxCombine];
+};
+
+template <class T>
+struct DevArr {
+  T *p = nullptr;
+  ~DevArr() {
+    if (p) hipFree(p);
+  }
+  int alloc(size_t count) {
+    if (hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess)
+      return set_error(GG_ERR_HIP, "device allocation of %zu bytes failed", count * sizeof(T));
+    return GG_OK;
+  }
+};
+
+struct PlanHolder {
+  gg_plan *p = nullptr;
+  ~PlanHolder() {
+    if (p) gg_plan_destroy(p);
+  }
+};
+
+struct MlpHolder {
+  gg_mlp *p = nullptr;
+  ~MlpHolder() {
+    if (p) gg_mlp_destroy(p);
+  }
+};
+
+}  // namespace
+
+extern "C" int gg_apply_move(int lines, int32_t *edges, int line0, int line1, int variant) {
+  if (!edges || line0 < 0 || line1 < 0 || line0 >= lines || line1 >= lines || line0 == line1)
+    return set_error(GG_ERR_INVALID, "bad move");
+  const int a = edges[2 * line0], b = edges[2 * line0 + 1], c = edges[2 * line1], d = edges[2 * line1 + 1];
+  edges[2 * line0] = a;
+  edges[2 * line0 + 1] = variant ? c : d;
+  edges[2 * line1] = variant ? b : c;
+  edges[2 * line1 + 1] = variant ? d : b;
+  return GG_OK;
+}
+
+extern "C" int gg_sa_features(int nodes, int degree, const int32_t *adj, int count,
+                              const int32_t *props, const float *vstat, float *feats) {
+  if (!adj || !props || !feats || nodes < 2 || degree < 1 || count < 1)
+    return set_error(GG_ERR_INVALID, "bad feature arguments");
+  for (size_t i = 0; i < (size_t)nodes * degree; ++i)
+    if (adj[i] < 0 || adj[i] >= nodes) return set_error(GG_ERR_INVALID, "adjacency entry out of range");
+  for (int i = 0; i < count; ++i)
+    for (int k = 0; k < 5; ++k) {
+      const int v = props[5 * i + k];
+      if (k < 4 ? (v < 0 || v >= nodes) : (v != 0 && v != 1))
+        return set_error(GG_ERR_INVALID, "proposal %d out of range", i);
+    }
+  int rc = gg_init(-1);
+  if (rc) return rc;
+  hipStream_t es = gg::engine_stream();
+  DevArr<int32_t> adj_d, props_d;
+  DevArr<float> vstat_d, feats_d;
+  if ((rc = adj_d.alloc((size_t)nodes * degree)) || (rc = props_d.alloc((size_t)count * 5)) ||
+      (rc = vstat_d.alloc((size_t)nodes * kVStat)) || (rc = feats_d.alloc((size_t)count * kFeat)))
+    return rc;
+  BSA_HIP(hipMemcpyAsync(adj_d.p, adj, (size_t)nodes * degree * 4, hipMemcpyHostToDevice, es));
+  BSA_HIP(hipMemcpyAsync(props_d.p, props, (size_t)count * 5 * 4, hipMemcpyHostToDevice, es));
+  if (vstat)
+    BSA_HIP(hipMemcpyAsync(vstat_d.p, vstat, (size_t)nodes * kVStat * 4, hipMemcpyHostToDevice, es));
+  else
+    BSA_HIP(hipMemsetAsync(vstat_d.p, 0, (size_t)nodes * kVStat * 4, es));
+  hipLaunchKernelGGL(gg_bsa_features, dim3(count), dim3(kBlock), 2 * (size_t)((nodes + 31) / 32) * 4, es,
+                     (const int32_t *)adj_d.p, nodes, degree, (const int32_t *)props_d.p,
+                     (const float *)vstat_d.p, feats_d.p);
+  BSA_HIP(hipGetLastError());
+  BSA_HIP(hipMemcpyAsync(feats, feats_d.p, (size_t)count * kFeat * 4, hipMemcpyDeviceToHost, es));
+  BSA_HIP(hipStreamSynchronize(es));
+  return GG_OK;
+}
+
+extern "C" int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa_params *P,
+                             gg_bsa_stats *stats, double *log_time, double *log_best_aspl,
+                             int32_t *log_best_diam, int32_t *accepted, int32_t *accepted_diam,
+                             uint64_t *accepted_sum, int accepted_capacity) {
+  if (!edges || !P || nodes < 8 || degree < 2 || ((int64_t)nodes * degree) % 2)
+    return set_error(GG_ERR_INVALID, "bad batched SA arguments");
+  if (P->combine < 1 || P->combine > kMaxCombine)
+    return set_error(GG_ERR_INVALID, "combine must be in 1..%d", kMaxCombine);
+  const int E = std::max(1, P->exact);
+  const bool use_sur = P->use_surrogate != 0;
+  const int NP = use_sur ? std::max(P->proposals, 1) : 0;
+  const int npmax = std::max(NP, E);
+  const double t0 = now_s();
+  double t_exact = 0, t_sur = 0, t_feat = 0, t_train = 0;
+  int rc = gg_init(-1);
+  if (rc) return rc;
+  SaGraph g(nodes, degree, edges);
+  if ((rc = g.init())) return rc;
+  std::vector<int32_t> best_edges(edges, edges + 2 * (size_t)g.lines);
+  Rng rng(P->seed);
+  hipStream_t es = gg::engine_stream();
+
+  gg_graph shape = {nodes, degree, nodes, 1, 0, GG_GRAPH_SYMMETRIC, g.adj.data(), nullptr};
+  PlanHolder plan;
+  plan.p = gg_plan_create_patched(E, &shape, 4 * P->combine);
+  if (!plan.p) return GG_ERR_HIP;
+  const int max_patch = 4 * P->combine;
+  std::vector<gg_result> res(E);
+  std::vector<int32_t> patches((size_t)E * max_patch * 2, -1);
+  rc = gg_plan_set_base(plan.p, g.adj.data());
+  if (!rc) rc = gg_plan_set_patches(plan.p, patches.data());
+  if (!rc) rc = gg_plan_run(plan.p, res.data());
+  if (rc) return rc;
+  const double npairs = ((double)nodes - 1) * nodes;
+  int cur_diam = res[0].diameter;
+  uint64_t cur_S = res[0].sum;
+  int best_diam = cur_diam;
+  uint64_t best_S = cur_S;
+  int64_t evals = 1, accepts = 0, step = 0, n_logged = 0, moves_acc = 0;
+  double to_target = -1;
+
+  // surrogate state
+  MlpHolder mlp;
+  VertexStats vs;
+  vs.maxd = std::min(std::max(P->hist_maxd, 2), 15);
+  const int cap = 16384;
+  const int bt = std::max(P->batch_train, 32);
+  DevArr<int32_t> props_d, idx_d;
+  DevArr<float> vstat_d, feats_d, replay_d, batch_d;
+  std::vector<double> by(cap);  // raw labels of the replay rows
+  int nbuf = 0, head = 0;
+  double scale = 0;
+  std::vector<double> corr_pred, corr_true;
+  if (use_sur) {
+    const int dims[5] = {kFeat, 256, 256, 256, 1};
+    mlp.p = gg_mlp_create(4, dims, P->seed ^ 0x5eedull);
+    if (!mlp.p) return GG_ERR_HIP;
+    if ((rc = props_d.alloc((size_t)npmax * 5)) || (rc = feats_d.alloc((size_t)npmax * kFeat)) ||
+        (rc = replay_d.alloc((size_t)cap * kFeat)) || (rc = batch_d.alloc((size_t)bt * kFeat)) ||
+        (rc = vstat_d.alloc((size_t)nodes * kVStat)) ||
+        (rc = idx_d.alloc(2 * (size_t)std::max(npmax, bt))))
+      return rc;
+    const double tf = now_s();
+    rc = vs.refresh(g);
+    if (!rc && hipMemcpy(vstat_d.p, vs.host.data(), vs.host.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+      rc = set_error(GG_ERR_HIP, "vertex statistics upload failed");
+    t_feat += now_s() - tf;
+    if (rc) return rc;
+  }
+  const size_t feat_lds = 2 * (size_t)((nodes + 31) / 32) * 4;
+
+  std::vector<Prop> props(npmax);
+  std::vector<int32_t> props_h((size_t)npmax * 5);
+  std::vector<float> pred(npmax, 0.0f);
+  std::vector<int> order(npmax);
+  std::vector<Cand> cand(E);
+  std::vector<int> cand_verts;
+  std::vector<int32_t> src_rows, dst_rows;
+  std::vector<int32_t> idx_h(2 * (size_t)std::max(npmax, bt));
+  std::vector<float> yb(bt);
+  int m_cur = P->combine;
+  int since_refresh = 0;
+  const double maxT = P->max_temp, minT = P->min_temp > 0 ? P->min_temp : 1e-12;
+
+  for (step = 0; step < P->max_steps; ++step) {
+    if (P->time_budget_s > 0 && now_s() - t0 > P->time_budget_s) break;
+    const double temp =
+        P->max_steps > 1 ? maxT * std::pow(minT / maxT, (double)step / (P->max_steps - 1)) : maxT;
+    const bool explore = !use_sur || step < P->warmup ||
+                         (P->explore_every > 0 && (step - P->warmup) % P->explore_every == P->explore_every - 1);
+    // --- proposals
+    double tp = now_s();
+    const int np = explore ? E : NP;
+    for (int i = 0; i < np; ++i) {
+      Prop &p = props[i];
+      if (explore) {
+        Slots s;
+        int tries = 0;
+        for (;;) {
+          g.draw(rng, p);
+          if (g.fresh(p) && g.slots(p, s)) break;
+          if (++tries > 100000) return set_error(GG_ERR_INVALID, "no valid 2-opt move found");
+        }
+        cand[i].nm = 1;
+        cand[i].prop[0] = i;
+        cand[i].sl[0] = s;
+      } else {
+        g.draw(rng, p);
+      }
+      int32_t *o = &props_h[(size_t)i * 5];
+      o[0] = p.a;
+      o[1] = p.b;
+      o[2] = p.c;
+      o[3] = p.d;
+      o[4] = p.r;
+    }
+    t_feat += now_s() - tp;
+    // --- features + ranking on the device
+    if (use_sur) {
+      const double ts = now_s();
+      BSA_HIP(hipMemcpyAsync(props_d.p, props_h.data(), (size_t)np * 5 * 4, hipMemcpyHostToDevice, es));
+      hipLaunchKernelGGL(gg_bsa_features, dim3(np), dim3(kBlock), feat_lds, es,
+                         (const int32_t *)gg_plan_base_device(plan.p), nodes, degree,
+                         (const int32_t *)props_d.p, (const float *)vstat_d.p, feats_d.p);
+      BSA_HIP(hipGetLastError());
+      BSA_HIP(hipStreamSynchronize(es));
+      if ((rc = gg::mlp_forward_device(mlp.p, feats_d.p, np, pred.data()))) return rc;
+      t_sur += now_s() - ts;
+    }
+    // --- candidates
+    int ncand = explore ? E : 0;
+    if (!explore) {
+      const double tc = now_s();
+      std::iota(order.begin(), order.begin() + np, 0);
+      const int want = std::min(np, E * m_cur * 4 + 64);
+      std::partial_sort(order.begin(), order.begin() + want, order.begin() + np,
+                        [&](int x, int y) { return pred[x] < pred[y]; });
+      for (int c = 0; c < E; ++c) cand[c].nm = 0;
+      // Greedy fill in rank order: candidate c takes up to m_cur valid moves
+      // that share no vertex; a move predicted to raise S closes a non-empty
+      // candidate (and may then start the next one alone).
+      int c = 0;
+      cand_verts.clear();
+      for (int oi = 0; oi < want && c < E; ++oi) {
+        const int i = order[oi];
+        const Prop &p = props[i];
+        if (cand[c].nm > 0 && pred[i] >= 0.0f) {
+          cand_verts.clear();
+          if (++c >= E) break;
+        }
+        bool clash = false;
+        for (int v : cand_verts) clash |= v == p.a || v == p.b || v == p.c || v == p.d;
+        Slots s;
+        if (clash || !g.fresh(p) || !g.slots(p, s)) continue;
+        Cand &C = cand[c];
+        C.prop[C.nm] = i;
+        C.sl[C.nm] = s;
+        ++C.nm;
+        cand_verts.insert(cand_verts.end(), {p.a, p.b, p.c, p.d});
+        if (C.nm >= m_cur) {
+          cand_verts.clear();
+          ++c;
+        }
+      }
+      ncand = std::min(E, c + (c < E && cand[c].nm > 0 ? 1 : 0));
+      t_feat += now_s() - tc;
+    }
+    // --- exact scoring of all candidates in one plan launch
+    std::fill(patches.begin(), patches.end(), -1);
+    for (int c = 0; c < ncand; ++c)
+      for (int k = 0; k < cand[c].nm; ++k)
+        for (int j = 0; j < 4; ++j) {
+          patches[((size_t)c * max_patch + 4 * k + j) * 2] = cand[c].sl[k].at[j];
+          patches[((size_t)c * max_patch + 4 * k + j) * 2 + 1] = cand[c].sl[k].val[j];
+        }
+    const double te = now_s();
+    rc = gg_plan_set_patches(plan.p, patches.data());
+    if (!rc) rc = gg_plan_run(plan.p, res.data());
+    t_exact += now_s() - te;
+    if (rc) return rc;
+    evals += ncand;
+    int bi = -1;
+    for (int c = 0; c < ncand; ++c) {
+      if (cand[c].nm == 0 || !res[c].connected || res[c].diameter >= nodes) continue;
+      if (bi < 0 || res[c].diameter < res[bi].diameter ||
+          (res[c].diameter == res[bi].diameter && res[c].sum < res[bi].sum))
+        bi = c;
+    }
+    // --- training rows: singles keep their exact dS; a merged candidate's
+    // members share its prediction error equally.
+    if (use_sur) {
+      const double ts = now_s();
+      src_rows.clear();
+      dst_rows.clear();
+      for (int c = 0; c < ncand; ++c) {
+        const Cand &C = cand[c];
+        if (C.nm == 0 || !res[c].connected || res[c].diameter >= nodes) continue;
+        const double dS = (double)res[c].sum - (double)cur_S;
+        double psum = 0;
+        for (int k = 0; k < C.nm; ++k) psum += pred[C.prop[k]];
+        const double resid = (dS - scale * psum) / C.nm;
+        for (int k = 0; k < C.nm; ++k) {
+          src_rows.push_back(C.prop[k]);
+          dst_rows.push_back(head);
+          by[head] = C.nm == 1 ? dS : scale * pred[C.prop[k]] + resid;
+          head = (head + 1) % cap;
+          nbuf = std::min(nbuf + 1, cap);
+        }
+        const double mag = std::fabs(dS) / C.nm + 1.0;
+        scale = scale == 0 ? mag : 0.995 * scale + 0.005 * mag;
+        if (explore && step >= P->warmup && C.nm == 1) {
+          corr_pred.push_back(pred[C.prop[0]]);
+          corr_true.push_back(dS);
+        }
+      }
+      const int rows = (int)src_rows.size();
+      if (rows) {
+        std::copy(src_rows.begin(), src_rows.end(), idx_h.begin());
+        std::copy(dst_rows.begin(), dst_rows.end(), idx_h.begin() + rows);
+        BSA_HIP(hipMemcpyAsync(idx_d.p, idx_h.data(), (size_t)2 * rows * 4, hipMemcpyHostToDevice, es));
+        hipLaunchKernelGGL(gg_bsa_copy_rows, dim3((rows * kFeat + 255) / 256), dim3(256), 0, es,
+                           (const float *)feats_d.p, (const int32_t *)idx_d.p, replay_d.p,
+                           (const int32_t *)idx_d.p + rows, rows);
+        BSA_HIP(hipGetLastError());
+      }
+      const double tt = now_s();
+      const int b = std::min(bt, nbuf);
+      for (int s = 0; s < P->train_steps && nbuf >= 32; ++s) {
+        for (int i = 0; i < b; ++i) {
+          const int j = (int)rng.below((uint32_t)nbuf);
+          idx_h[i] = j;
+          yb[i] = (float)(by[j] / scale);
+        }
+        BSA_HIP(hipMemcpyAsync(idx_d.p, idx_h.data(), (size_t)b * 4, hipMemcpyHostToDevice, es));
+        hipLaunchKernelGGL(gg_bsa_copy_rows, dim3((b * kFeat + 255) / 256), dim3(256), 0, es,
+                           (const float *)replay_d.p, (const int32_t *)idx_d.p, batch_d.p,
+                           (const int32_t *)nullptr, b);
+        BSA_HIP(hipGetLastError());
+        BSA_HIP(hipStreamSynchronize(es));
+        float loss = 0;
+        if ((rc = gg::mlp_train_device(mlp.p, batch_d.p, yb.data(), b, P->lr, &loss))) return rc;
+      }
+      BSA_HIP(hipStreamSynchronize(es));
+      t_train += now_s() - tt;
+      t_sur += now_s() - ts;
+    }
+    // --- the reference's accept rule (general/sa.c:371-411) on the best candidate
+    bool improved = false;
+    if (bi >= 0) {
+      const int nd = res[bi].diameter;
+      const uint64_t nS = res[bi].sum;
+      bool acc;
+      if (nd < cur_diam) {
+        acc = true;
+      } else if (nd > cur_diam) {
+        acc = false;
+      } else if (nS <= cur_S) {
+        acc = true;
+      } else {
+        const double diff = -((double)nS - (double)cur_S);
+        acc = std::exp(diff / temp) > (rng.below(1u << 30) + 1.0) / ((double)(1u << 30) + 2.0);
+      }
+      improved = nd < cur_diam || (nd == cur_diam && nS < cur_S);
+      if (acc) {
+        const Cand &C = cand[bi];
+        for (int k = 0; k < C.nm; ++k) {
+          const Prop &p = props[C.prop[k]];
+          g.apply(p, C.sl[k]);
+          if (accepted && n_logged < accepted_capacity) {
+            accepted[3 * n_logged] = p.t0;
+            accepted[3 * n_logged + 1] = p.t1;
+            accepted[3 * n_logged + 2] = p.r;
+            const bool last = k + 1 == C.nm;
+            if (accepted_diam) accepted_diam[n_logged] = last ? nd : -1;
+            if (accepted_sum) accepted_sum[n_logged] = last ? nS : 0;
+            ++n_logged;
+          }
+        }
+        moves_acc += C.nm;
+        if ((rc = gg_plan_set_base(plan.p, g.adj.data()))) return rc;
+        cur_diam = nd;
+        cur_S = nS;
+        ++accepts;
+        if (cur_diam < best_diam || (cur_diam == best_diam && cur_S < best_S)) {
+          best_diam = cur_diam;
+          best_S = cur_S;
+          std::copy(edges, edges + 2 * (size_t)g.lines, best_edges.begin());
+        }
+        if (use_sur && ++since_refresh >= std::max(1, P->feature_refresh)) {
+          since_refresh = 0;
+          const double tf = now_s();
+          rc = vs.refresh(g);
+          if (!rc && hipMemcpy(vstat_d.p, vs.host.data(), vs.host.size() * 4, hipMemcpyHostToDevice) !=
+                         hipSuccess)
+            rc = set_error(GG_ERR_HIP, "vertex statistics upload failed");
+          t_feat += now_s() - tf;
+          if (rc) return rc;
+        }
+      }
+    }
+    if (!explore) m_cur = improved ? std::min(P->combine, m_cur + 1) : std::max(1, m_cur / 2);
+    const double best_aspl = (double)best_S / npairs;
+    if (log_time && step < P->log_capacity) {
+      log_time[step] = now_s() - t0;
+      if (log_best_aspl) log_best_aspl[step] = best_aspl;
+      if (log_best_diam) log_best_diam[step] = best_diam;
+    }
+    if (P->target_aspl > 0 && best_aspl <= P->target_aspl) {
+      to_target = now_s() - t0;
+      ++step;
+      break;
+    }
+  }
+  std::copy(best_edges.begin(), best_edges.end(), edges);
+  if (stats) {
+    stats->steps = step;
+    stats->accepts = accepts;
+    stats->evaluations = evals;
+    stats->best_diameter = best_diam;
+    stats->best_aspl = (double)best_S / npairs;
+    stats->seconds_total = now_s() - t0;
+    stats->seconds_exact = t_exact;
+    stats->seconds_surrogate = t_sur;
+    stats->seconds_features = t_feat;
+    stats->seconds_to_target = to_target;
+    stats->surrogate_rank_corr = spearman(corr_pred, corr_true);
+    stats->moves_accepted = moves_acc;
+    stats->seconds_train = t_train;
+  }
+  return GG_OK;
+}

@@ -1,0 +1,19 @@
+// gg_device.h -- HIP-side internals shared by the .hip translation units
+// (streams and device-pointer entry points that the C ABI does not expose).
+#ifndef GG_DEVICE_H
+#define GG_DEVICE_H
+#include <hip/hip_runtime.h>
+
+struct gg_mlp;
+
+namespace gg {
+// The evaluator's stream (adjacency uploads, plans, histograms run on it).
+hipStream_t engine_stream();
+// Forward pass on m rows already in device memory (row-major, in_dim floats
+// each); copies the m predictions to y_host and waits for them.
+int mlp_forward_device(gg_mlp *M, const float *x_dev, int m, float *y_host);
+// One Adam step on m device-resident rows with host targets t_host.
+int mlp_train_device(gg_mlp *M, const float *x_dev, const float *t_host, int m, float lr,
+                     float *loss);
+}  // namespace gg
+#endif

@@ -48,6 +48,7 @@
 #include "gg_eval.h"
 #include "gg_host.h"
 #include "gg_internal.h"
+#include "gg_device.h"
 
 // ---------------------------------------------------------------------------
 // Device-side descriptors.
@@ -777,6 +778,8 @@ struct Engine {
 static Engine E;
 
 Engine &engine() { return E; }
+
+hipStream_t engine_stream() { return E.stream; }
 
 int init(int device) {
   if (E.ready) return GG_OK;

@@ -12,37 +12,14 @@
 #include "gg_eval.h"
 #include "gg_host.h"
 #include "gg_internal.h"
+#include "gg_sa_common.h"
+
+using gg::EdgeSet;
+using gg::ekey;
+using gg::now_s;
+using gg::Rng;
 
 namespace {
-
-// xoshiro256** seeded through splitmix64: the generators' own RNG (the SA
-// drivers use libc random() instead, to follow the reference's draws).
-struct Rng {
-  uint64_t s[4];
-  explicit Rng(uint64_t seed) {
-    for (int i = 0; i < 4; ++i) {
-      seed += 0x9E3779B97F4A7C15ull;
-      uint64_t z = seed;
-      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-      z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-      s[i] = z ^ (z >> 31);
-    }
-  }
-  static uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
-  uint64_t next() {
-    const uint64_t r = rotl(s[1] * 5, 7) * 9;
-    const uint64_t t = s[1] << 17;
-    s[2] ^= s[0];
-    s[3] ^= s[1];
-    s[1] ^= s[2];
-    s[0] ^= s[3];
-    s[2] ^= t;
-    s[3] = rotl(s[3], 45);
-    return r;
-  }
-  // Uniform in [0, m) (Lemire's multiply-shift; bias < 2^-32 for our m).
-  uint32_t below(uint32_t m) { return (uint32_t)(((next() >> 32) * (uint64_t)m) >> 32); }
-};
 
 // Fixed-degree adjacency under construction.
 struct Adj {
@@ -375,47 +352,6 @@ void moore(int nodes, int degree, int *low_diam, double *low_aspl) {
   *low_aspl = aspl;
 }
 
-inline uint64_t ekey(int a, int b) {
-  return a < b ? ((uint64_t)(uint32_t)a << 32) | (uint32_t)b : ((uint64_t)(uint32_t)b << 32) | (uint32_t)a;
-}
-
-struct EdgeSet {  // open-addressing set of unordered edges
-  std::vector<uint64_t> slot;
-  uint64_t mask;
-  explicit EdgeSet(size_t n_edges) {
-    size_t cap = 16;
-    while (cap < n_edges * 4) cap <<= 1;
-    slot.assign(cap, ~0ull);
-    mask = cap - 1;
-  }
-  static uint64_t h(uint64_t k) {
-    k ^= k >> 33;
-    k *= 0xff51afd7ed558ccdull;
-    k ^= k >> 33;
-    return k;
-  }
-  bool has(uint64_t k) const {
-    for (uint64_t i = h(k) & mask;; i = (i + 1) & mask) {
-      if (slot[i] == k) return true;
-      if (slot[i] == ~0ull) return false;
-    }
-  }
-  void add(uint64_t k) {
-    uint64_t i = h(k) & mask;
-    while (slot[i] != ~0ull && slot[i] != ~1ull) i = (i + 1) & mask;
-    slot[i] = k;
-  }
-  void remove(uint64_t k) {
-    for (uint64_t i = h(k) & mask;; i = (i + 1) & mask) {
-      if (slot[i] == k) {
-        slot[i] = ~1ull;  // tombstone
-        return;
-      }
-      if (slot[i] == ~0ull) return;
-    }
-  }
-};
-
 struct Move {  // what a 2-opt changed, for restore
   int line[2];
   int old_edge[4];
@@ -524,10 +460,6 @@ int device_eval(void *, int nodes, int degree, const int32_t *adj, int32_t *diam
   return GG_OK;
 }
 
-double now_s() {
-  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
-}
-
 }  // namespace
 
 extern "C" int gg_sa_general(int nodes, int degree, int32_t *edges, const gg_sa_params *P,
@@ -629,406 +561,3 @@ extern "C" int gg_sa_general(int nodes, int degree, int32_t *edges, const gg_sa_
   return GG_OK;
 }
 
-// ---------------------------------------------------------------------------
-// Batched, surrogate-ranked SA (see include/gg_host.h).
-
-#include "gg_mlp.h"
-
-namespace {
-
-constexpr int kFeat = 64;
-
-struct BMove {
-  int t0, t1, r;
-  int a, b, c, d;
-  int at[4], val[4];
-};
-
-struct BatchState {
-  int n, d, lines;
-  int32_t *edges;
-  std::vector<int32_t> adj;
-  EdgeSet set;
-  BatchState(int n_, int d_, int32_t *e)
-      : n(n_), d(d_), lines(n_ * d_ / 2), edges(e), adj((size_t)n_ * d_), set((size_t)n_ * d_ / 2) {
-    gg_edges_to_adj(n, lines, d, edges, adj.data());
-    for (int i = 0; i < lines; ++i) set.add(ekey(edges[2 * i], edges[2 * i + 1]));
-  }
-  int find(int y, int v) const {
-    for (int x = 0; x < d; ++x)
-      if (adj[(size_t)y * d + x] == v) return x;
-    return -1;
-  }
-  // A random valid 2-opt move: (a,b),(c,d) -> (a,d),(c,b) [r=0] or (a,c),(b,d) [r=1].
-  bool draw(Rng &rng, BMove &m) const {
-    m.t0 = (int)rng.below(lines);
-    m.t1 = (int)rng.below(lines);
-    m.r = (int)rng.below(2);
-    if (m.t0 == m.t1) return false;
-    m.a = edges[2 * m.t0];
-    m.b = edges[2 * m.t0 + 1];
-    m.c = edges[2 * m.t1];
-    m.d = edges[2 * m.t1 + 1];
-    if (m.a == m.c || m.a == m.d || m.b == m.c || m.b == m.d) return false;
-    const int p1 = m.a, q1 = m.r ? m.c : m.d, p2 = m.r ? m.b : m.c, q2 = m.r ? m.d : m.b;
-    if (set.has(ekey(p1, q1)) || set.has(ekey(p2, q2))) return false;
-    const int x0 = find(m.a, m.b), x1 = find(m.b, m.a), x2 = find(m.c, m.d), x3 = find(m.d, m.c);
-    if (x0 < 0 || x1 < 0 || x2 < 0 || x3 < 0) return false;
-    m.at[0] = m.a * d + x0;
-    m.at[1] = m.b * d + x1;
-    m.at[2] = m.c * d + x2;
-    m.at[3] = m.d * d + x3;
-    if (m.r == 0) {
-      m.val[0] = m.d;
-      m.val[1] = m.c;
-      m.val[2] = m.b;
-      m.val[3] = m.a;
-    } else {
-      m.val[0] = m.c;
-      m.val[1] = m.d;
-      m.val[2] = m.a;
-      m.val[3] = m.b;
-    }
-    return true;
-  }
-  void apply(const BMove &m) {
-    for (int k = 0; k < 4; ++k) adj[m.at[k]] = m.val[k];
-    set.remove(ekey(m.a, m.b));
-    set.remove(ekey(m.c, m.d));
-    int e0a = m.a, e0b = m.r ? m.c : m.d, e1a = m.r ? m.b : m.c, e1b = m.r ? m.d : m.b;
-    set.add(ekey(e0a, e0b));
-    set.add(ekey(e1a, e1b));
-    edges[2 * m.t0] = e0a;
-    edges[2 * m.t0 + 1] = e0b;
-    edges[2 * m.t1] = e1a;
-    edges[2 * m.t1 + 1] = e1b;
-  }
-  int common(int x, int y) const {
-    int c = 0;
-    const int32_t *rx = &adj[(size_t)x * d], *ry = &adj[(size_t)y * d];
-    for (int i = 0; i < d; ++i)
-      for (int j = 0; j < d; ++j) c += (rx[i] == ry[j]);
-    return c;
-  }
-};
-
-// Per-vertex statistics from distance histograms.
-struct VertexStats {
-  int maxd = 8;
-  std::vector<int32_t> hist;  // [n][maxd+1]
-  std::vector<float> S, ecc, nbr;
-  double S_mean = 1, diam = 1;
-  int refresh(const BatchState &st) {
-    const int n = st.n;
-    hist.assign((size_t)n * (maxd + 1), 0);
-    int rc = gg_distance_histogram(n, st.d, st.adj.data(), maxd, hist.data());
-    if (rc) return rc;
-    S.assign(n, 0);
-    ecc.assign(n, 0);
-    double tot = 0, dm = 1;
-    for (int v = 0; v < n; ++v) {
-      const int32_t *h = &hist[(size_t)v * (maxd + 1)];
-      double s = 0;
-      int e = 0;
-      for (int k = 1; k <= maxd; ++k) {
-        s += (double)k * h[k];
-        if (h[k]) e = k;
-      }
-      S[v] = (float)s;
-      ecc[v] = (float)e;
-      tot += s;
-      dm = std::max(dm, (double)e);
-    }
-    S_mean = std::max(tot / n, 1.0);
-    diam = dm;
-    nbr.assign(n, 0);
-    for (int v = 0; v < n; ++v) {
-      double acc = 0;
-      for (int j = 0; j < st.d; ++j) acc += S[st.adj[(size_t)v * st.d + j]] / S_mean - 1.0;
-      nbr[v] = (float)(acc / st.d);
-    }
-    return GG_OK;
-  }
-  void features(const BatchState &st, const BMove &m, float *f) const {
-    const int n = st.n;
-    std::fill(f, f + kFeat, 0.0f);
-    const int ends[4] = {m.a, m.b, m.c, m.d};
-    int k = 0;
-    for (int e = 0; e < 4; ++e) {
-      const int v = ends[e];
-      const int32_t *h = &hist[(size_t)v * (maxd + 1)];
-      for (int j = 1; j <= 7; ++j) f[k++] = j <= maxd ? (float)h[j] / n : 0.0f;
-      f[k++] = (float)(S[v] / S_mean - 1.0);
-      f[k++] = (float)(ecc[v] / diam);
-    }
-    const int p1 = m.a, q1 = m.r ? m.c : m.d, p2 = m.r ? m.b : m.c, q2 = m.r ? m.d : m.b;
-    const int px[4] = {m.a, m.c, p1, p2}, py[4] = {m.b, m.d, q1, q2};
-    for (int q = 0; q < 4; ++q) f[k++] = (float)st.common(px[q], py[q]) / st.d;
-    for (int q = 0; q < 4; ++q) f[k++] = (float)((S[px[q]] + S[py[q]]) / (2 * S_mean) - 1.0);
-    f[k++] = (float)m.r;
-    for (int e = 0; e < 4; ++e) f[k++] = nbr[ends[e]];
-  }
-};
-
-double spearman(std::vector<double> x, std::vector<double> y) {
-  const size_t n = x.size();
-  if (n < 3) return 0;
-  auto ranks = [&](std::vector<double> &v) {
-    std::vector<size_t> idx(n);
-    std::iota(idx.begin(), idx.end(), 0);
-    std::sort(idx.begin(), idx.end(), [&](size_t i, size_t j) { return v[i] < v[j]; });
-    std::vector<double> r(n);
-    for (size_t i = 0; i < n; ++i) r[idx[i]] = (double)i;
-    return r;
-  };
-  std::vector<double> rx = ranks(x), ry = ranks(y);
-  double mx = (n - 1) / 2.0, sxy = 0, sxx = 0, syy = 0;
-  for (size_t i = 0; i < n; ++i) {
-    sxy += (rx[i] - mx) * (ry[i] - mx);
-    sxx += (rx[i] - mx) * (rx[i] - mx);
-    syy += (ry[i] - mx) * (ry[i] - mx);
-  }
-  return (sxx > 0 && syy > 0) ? sxy / std::sqrt(sxx * syy) : 0;
-}
-
-}  // namespace
-
-extern "C" int gg_apply_move(int lines, int32_t *edges, int line0, int line1, int variant) {
-  if (!edges || line0 < 0 || line1 < 0 || line0 >= lines || line1 >= lines || line0 == line1)
-    return set_error(GG_ERR_INVALID, "bad move");
-  const int a = edges[2 * line0], b = edges[2 * line0 + 1], c = edges[2 * line1], d = edges[2 * line1 + 1];
-  edges[2 * line0] = a;
-  edges[2 * line0 + 1] = variant ? c : d;
-  edges[2 * line1] = variant ? b : c;
-  edges[2 * line1 + 1] = variant ? d : b;
-  return GG_OK;
-}
-
-extern "C" int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa_params *P,
-                             gg_bsa_stats *stats, double *log_time, double *log_best_aspl,
-                             int32_t *log_best_diam, int32_t *accepted, int32_t *accepted_diam,
-                             uint64_t *accepted_sum, int accepted_capacity) {
-  if (!edges || !P || nodes < 8 || degree < 2 || ((int64_t)nodes * degree) % 2)
-    return set_error(GG_ERR_INVALID, "bad batched SA arguments");
-  const int E = std::max(1, P->exact);
-  const int NP = std::max(E, P->use_surrogate ? P->proposals : E);
-  const int explore = P->use_surrogate ? std::min(std::max(0, P->explore), E) : E;
-  const double t0 = now_s();
-  double t_exact = 0, t_sur = 0, t_feat = 0;
-  BatchState st(nodes, degree, edges);
-  std::vector<int32_t> best_edges(edges, edges + 2 * (size_t)st.lines);
-  Rng rng(P->seed);
-
-  gg_graph shape = {nodes, degree, nodes, 1, 0, GG_GRAPH_SYMMETRIC, st.adj.data(), nullptr};
-  gg_plan *plan = gg_plan_create_patched(E, &shape, 4);
-  if (!plan) return GG_ERR_HIP;
-  std::vector<gg_result> res(E);
-  std::vector<int32_t> patches((size_t)E * 4 * 2, -1);
-  int rc = gg_plan_set_base(plan, st.adj.data());
-  // the current graph: one replay with no patches
-  if (!rc) rc = gg_plan_set_patches(plan, patches.data());
-  if (!rc) rc = gg_plan_run(plan, res.data());
-  if (rc) {
-    gg_plan_destroy(plan);
-    return rc;
-  }
-  const double npairs = ((double)nodes - 1) * nodes;
-  int cur_diam = res[0].diameter;
-  uint64_t cur_S = res[0].sum;
-  int best_diam = cur_diam;
-  uint64_t best_S = cur_S;
-  int64_t evals = 1, accepts = 0, step = 0, n_acc_logged = 0;
-  double to_target = -1;
-
-  gg_mlp *mlp = nullptr;
-  VertexStats vs;
-  vs.maxd = std::min(std::max(P->hist_maxd, 2), 15);
-  if (P->use_surrogate) {
-    const int dims[5] = {kFeat, 256, 256, 256, 1};
-    mlp = gg_mlp_create(4, dims, P->seed ^ 0x5eedull);
-    if (!mlp) {
-      gg_plan_destroy(plan);
-      return GG_ERR_HIP;
-    }
-    double tf = now_s();
-    rc = vs.refresh(st);
-    t_feat += now_s() - tf;
-    if (rc) {
-      gg_mlp_destroy(mlp);
-      gg_plan_destroy(plan);
-      return rc;
-    }
-  }
-  // training ring buffer of (features, raw dS)
-  const int cap = 16384;
-  std::vector<float> bx((size_t)cap * kFeat);
-  std::vector<double> by(cap);
-  int nbuf = 0, head = 0;
-  double scale = 0;
-  std::vector<double> corr_hist;
-
-  std::vector<BMove> props(NP);
-  std::vector<float> feats((size_t)NP * kFeat), pred(NP);
-  std::vector<int> chosen;
-  int since_refresh = 0;
-  const double maxT = P->max_temp, minT = P->min_temp > 0 ? P->min_temp : 1e-12;
-  for (step = 0; step < P->max_steps; ++step) {
-    if (P->time_budget_s > 0 && now_s() - t0 > P->time_budget_s) break;
-    const double temp =
-        P->max_steps > 1 ? maxT * std::pow(minT / maxT, (double)step / (P->max_steps - 1)) : maxT;
-    // proposals
-    for (int i = 0; i < NP && !rc; ++i) {
-      int tries = 0;
-      while (!rc && !st.draw(rng, props[i]))
-        if (++tries > 100000) rc = set_error(GG_ERR_INVALID, "no valid 2-opt move found");
-    }
-    if (rc) break;
-    chosen.clear();
-    if (mlp) {
-      double tf = now_s();
-      for (int i = 0; i < NP; ++i) vs.features(st, props[i], &feats[(size_t)i * kFeat]);
-      t_feat += now_s() - tf;
-      double ts = now_s();
-      rc = gg_mlp_forward(mlp, feats.data(), NP, pred.data());
-      t_sur += now_s() - ts;
-      if (rc) break;
-      std::vector<int> order(NP);
-      std::iota(order.begin(), order.end(), 0);
-      const int top = E - explore;
-      std::partial_sort(order.begin(), order.begin() + top, order.end(),
-                        [&](int x, int y) { return pred[x] < pred[y]; });
-      for (int i = 0; i < top; ++i) chosen.push_back(order[i]);
-      // exploration picks from the rest, without replacement
-      for (int i = 0; i < explore; ++i) {
-        int j = top + (int)rng.below((uint32_t)(NP - top - i));
-        std::swap(order[top + i], order[j]);
-        chosen.push_back(order[top + i]);
-      }
-    } else {
-      for (int i = 0; i < E; ++i) chosen.push_back(i);
-    }
-    for (int i = 0; i < E; ++i) {
-      const BMove &m = props[chosen[i]];
-      for (int k = 0; k < 4; ++k) {
-        patches[((size_t)i * 4 + k) * 2] = m.at[k];
-        patches[((size_t)i * 4 + k) * 2 + 1] = m.val[k];
-      }
-    }
-    double te = now_s();
-    rc = gg_plan_set_patches(plan, patches.data());
-    if (!rc) rc = gg_plan_run(plan, res.data());
-    t_exact += now_s() - te;
-    if (rc) break;
-    evals += E;
-    // best exact candidate
-    int bi = -1;
-    for (int i = 0; i < E; ++i) {
-      if (!res[i].connected || res[i].diameter >= nodes) continue;
-      if (bi < 0 || res[i].diameter < res[bi].diameter ||
-          (res[i].diameter == res[bi].diameter && res[i].sum < res[bi].sum))
-        bi = i;
-    }
-    // training data + ranking quality
-    if (mlp) {
-      std::vector<double> pv, ev;
-      for (int i = 0; i < E; ++i) {
-        if (!res[i].connected) continue;
-        const double dS = (double)res[i].sum - (double)cur_S;
-        std::memcpy(&bx[(size_t)head * kFeat], &feats[(size_t)chosen[i] * kFeat], kFeat * 4);
-        by[head] = dS;
-        head = (head + 1) % cap;
-        nbuf = std::min(nbuf + 1, cap);
-        scale = scale == 0 ? std::fabs(dS) + 1 : 0.99 * scale + 0.01 * (std::fabs(dS) + 1);
-        pv.push_back(pred[chosen[i]]);
-        ev.push_back(dS);
-      }
-      corr_hist.push_back(spearman(pv, ev));
-      if (corr_hist.size() > 64) corr_hist.erase(corr_hist.begin());
-      double ts = now_s();
-      const int bt = std::min(std::max(P->batch_train, 32), nbuf);
-      std::vector<float> xb((size_t)bt * kFeat), yb(bt);
-      for (int s = 0; s < P->train_steps && nbuf >= 32; ++s) {
-        for (int i = 0; i < bt; ++i) {
-          const int j = (int)rng.below((uint32_t)nbuf);
-          std::memcpy(&xb[(size_t)i * kFeat], &bx[(size_t)j * kFeat], kFeat * 4);
-          yb[i] = (float)(by[j] / scale);
-        }
-        float loss = 0;
-        if ((rc = gg_mlp_train_step(mlp, xb.data(), yb.data(), bt, P->lr, &loss))) break;
-      }
-      t_sur += now_s() - ts;
-      if (rc) break;
-    }
-    // the reference's accept rule (general/sa.c:371-411) on the best candidate
-    if (bi >= 0) {
-      const int nd = res[bi].diameter;
-      const uint64_t nS = res[bi].sum;
-      bool acc;
-      if (nd < cur_diam) acc = true;
-      else if (nd > cur_diam) acc = false;
-      else if (nS <= cur_S) acc = true;
-      else {
-        const double diff = -((double)nS - (double)cur_S);
-        acc = std::exp(diff / temp) > (rng.below(1u << 30) + 1.0) / ((double)(1u << 30) + 2.0);
-      }
-      if (acc) {
-        const BMove &m = props[chosen[bi]];
-        st.apply(m);
-        if ((rc = gg_plan_set_base(plan, st.adj.data()))) break;
-        cur_diam = nd;
-        cur_S = nS;
-        ++accepts;
-        if (accepted && n_acc_logged < accepted_capacity) {
-          accepted[3 * n_acc_logged] = m.t0;
-          accepted[3 * n_acc_logged + 1] = m.t1;
-          accepted[3 * n_acc_logged + 2] = m.r;
-          if (accepted_diam) accepted_diam[n_acc_logged] = nd;
-          if (accepted_sum) accepted_sum[n_acc_logged] = nS;
-          ++n_acc_logged;
-        }
-        if (cur_diam < best_diam || (cur_diam == best_diam && cur_S < best_S)) {
-          best_diam = cur_diam;
-          best_S = cur_S;
-          std::copy(edges, edges + 2 * (size_t)st.lines, best_edges.begin());
-        }
-        if (mlp && ++since_refresh >= std::max(1, P->feature_refresh)) {
-          since_refresh = 0;
-          double tf = now_s();
-          rc = vs.refresh(st);
-          t_feat += now_s() - tf;
-          if (rc) break;
-        }
-      }
-    }
-    const double best_aspl = (double)best_S / npairs;
-    if (log_time && step < P->log_capacity) {
-      log_time[step] = now_s() - t0;
-      if (log_best_aspl) log_best_aspl[step] = best_aspl;
-      if (log_best_diam) log_best_diam[step] = best_diam;
-    }
-    if (P->target_aspl > 0 && best_aspl <= P->target_aspl) {
-      to_target = now_s() - t0;
-      ++step;
-      break;
-    }
-  }
-  if (mlp) gg_mlp_destroy(mlp);
-  gg_plan_destroy(plan);
-  std::copy(best_edges.begin(), best_edges.end(), edges);
-  if (stats) {
-    stats->steps = step;
-    stats->accepts = accepts;
-    stats->evaluations = evals;
-    stats->best_diameter = best_diam;
-    stats->best_aspl = (double)best_S / npairs;
-    stats->seconds_total = now_s() - t0;
-    stats->seconds_exact = t_exact;
-    stats->seconds_surrogate = t_sur;
-    stats->seconds_features = t_feat;
-    stats->seconds_to_target = to_target;
-    double c = 0;
-    for (double x : corr_hist) c += x;
-    stats->surrogate_rank_corr = corr_hist.empty() ? 0 : c / corr_hist.size();
-  }
-  return rc;
-}

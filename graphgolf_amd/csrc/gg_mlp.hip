@@ -20,6 +20,7 @@
 
 #include "gg_eval.h"
 #include "gg_internal.h"
+#include "gg_device.h"
 #include "gg_mlp.h"
 
 using gg::set_error;
@@ -140,6 +141,12 @@ __global__ void gg_mlp_adam(float *__restrict__ p, const float *__restrict__ g, 
   p[i] -= lr * (a / c1) / (sqrtf(b / c2) + eps);
 }
 
+// y[i] = Z[i * ld] for i < m (the network's single output column).
+__global__ void gg_mlp_col0(const float *__restrict__ Z, int ld, int m, float *__restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) y[i] = Z[(size_t)i * ld];
+}
+
 // ---------------------------------------------------------------------------
 
 namespace {
@@ -168,6 +175,7 @@ struct gg_mlp {
   float *dA = nullptr, *dB = nullptr;  // [Mp][maxw] gradient scratch
   uint16_t *dAb = nullptr;
   float *x_f = nullptr, *t_f = nullptr, *loss = nullptr;
+  float *y_f = nullptr;              // [Mp] contiguous predictions
   int maxw = 0;
 };
 
@@ -191,10 +199,12 @@ static int mlp_reserve(gg_mlp *M, int m) {
   hipFree(M->dAb);
   hipFree(M->x_f);
   hipFree(M->t_f);
+  hipFree(M->y_f);
   M->cap_m = 0;
   MLP_HIP(hipMalloc(&M->x_b, (size_t)Mp * M->in_dim * 2));
   MLP_HIP(hipMalloc(&M->x_f, (size_t)Mp * M->in_dim * 4));
   MLP_HIP(hipMalloc(&M->t_f, (size_t)Mp * 4));
+  MLP_HIP(hipMalloc(&M->y_f, (size_t)Mp * 4));
   for (const Layer &L : M->layers) {
     float *z;
     uint16_t *h;
@@ -281,6 +291,45 @@ static int refresh_bf16(gg_mlp *M) {
   return GG_OK;
 }
 
+// Output column 0 of the last forward pass -> y (host), contiguous.
+static int copy_predictions(gg_mlp *M, int m, float *y) {
+  const Layer &Lo = M->layers.back();
+  hipLaunchKernelGGL(gg_mlp_col0, dim3((m + 255) / 256), dim3(256), 0, M->stream, M->Z.back(), Lo.out_p,
+                     m, M->y_f);
+  MLP_HIP(hipGetLastError());
+  MLP_HIP(hipMemcpyAsync(y, M->y_f, (size_t)m * 4, hipMemcpyDeviceToHost, M->stream));
+  MLP_HIP(hipStreamSynchronize(M->stream));
+  return GG_OK;
+}
+
+// Forward + backward for rows x (host or device) and host targets t.
+static int mlp_gradient(gg_mlp *M, const float *x, bool device_x, const float *t, int m, float *loss,
+                        float *grad) {
+  int rc = mlp_forward(M, x, m, device_x);
+  if (rc) return rc;
+  MLP_HIP(hipMemcpyAsync(M->t_f, t, (size_t)m * 4, hipMemcpyHostToDevice, M->stream));
+  if ((rc = mlp_backward(M, m))) return rc;
+  if (grad) MLP_HIP(hipMemcpyAsync(grad, M->g, (size_t)M->n_params * 4, hipMemcpyDeviceToHost, M->stream));
+  if (loss) MLP_HIP(hipMemcpyAsync(loss, M->loss, 4, hipMemcpyDeviceToHost, M->stream));
+  MLP_HIP(hipStreamSynchronize(M->stream));
+  return GG_OK;
+}
+
+static int mlp_train(gg_mlp *M, const float *x, bool device_x, const float *t, int m, float lr,
+                     float *loss) {
+  int rc = mlp_gradient(M, x, device_x, t, m, loss, nullptr);
+  if (rc) return rc;
+  ++M->step;
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float c1 = 1.0f - std::pow(b1, (float)M->step), c2 = 1.0f - std::pow(b2, (float)M->step);
+  hipLaunchKernelGGL(gg_mlp_adam, dim3((M->n_params + 255) / 256), dim3(256), 0, M->stream, M->p, M->g,
+                     M->m1, M->m2, M->n_params, lr, b1, b2, 1e-8f, c1, c2);
+  MLP_HIP(hipGetLastError());
+  if ((rc = refresh_bf16(M))) return rc;
+  MLP_HIP(hipStreamSynchronize(M->stream));
+  return GG_OK;
+}
+
 extern "C" {
 
 gg_mlp *gg_mlp_create(int n_layers, const int *dims, uint64_t seed) {
@@ -363,7 +412,7 @@ void gg_mlp_destroy(gg_mlp *M) {
   for (uint16_t *h : M->Hb) hipFree(h);
   for (void *q : {(void *)M->p, (void *)M->g, (void *)M->m1, (void *)M->m2, (void *)M->pb,
                   (void *)M->x_b, (void *)M->dA, (void *)M->dB, (void *)M->dAb, (void *)M->x_f,
-                  (void *)M->t_f, (void *)M->loss})
+                  (void *)M->t_f, (void *)M->loss, (void *)M->y_f})
     if (q) hipFree(q);
   if (M->stream) hipStreamDestroy(M->stream);
   delete M;
@@ -402,37 +451,35 @@ int gg_mlp_forward(gg_mlp *M, const float *x, int m, float *y) {
   if (!M || !x || !y || m < 1) return set_error(GG_ERR_INVALID, "bad arguments");
   int rc = mlp_forward(M, x, m, false);
   if (rc) return rc;
-  const Layer &Lo = M->layers.back();
-  MLP_HIP(hipMemcpy2DAsync(y, sizeof(float), M->Z.back(), (size_t)Lo.out_p * 4, sizeof(float), m,
-                           hipMemcpyDeviceToHost, M->stream));
-  MLP_HIP(hipStreamSynchronize(M->stream));
-  return GG_OK;
+  return copy_predictions(M, m, y);
 }
 
 int gg_mlp_gradient(gg_mlp *M, const float *x, const float *t, int m, float *loss, float *grad) {
   if (!M || !x || !t || m < 1) return set_error(GG_ERR_INVALID, "bad arguments");
-  int rc = mlp_forward(M, x, m, false);
-  if (rc) return rc;
-  MLP_HIP(hipMemcpyAsync(M->t_f, t, (size_t)m * 4, hipMemcpyHostToDevice, M->stream));
-  if ((rc = mlp_backward(M, m))) return rc;
-  if (grad) MLP_HIP(hipMemcpyAsync(grad, M->g, (size_t)M->n_params * 4, hipMemcpyDeviceToHost, M->stream));
-  if (loss) MLP_HIP(hipMemcpyAsync(loss, M->loss, 4, hipMemcpyDeviceToHost, M->stream));
-  MLP_HIP(hipStreamSynchronize(M->stream));
-  return GG_OK;
+  return mlp_gradient(M, x, false, t, m, loss, grad);
 }
 
 int gg_mlp_train_step(gg_mlp *M, const float *x, const float *t, int m, float lr, float *loss) {
-  int rc = gg_mlp_gradient(M, x, t, m, loss, nullptr);
-  if (rc) return rc;
-  ++M->step;
-  const float b1 = 0.9f, b2 = 0.999f;
-  const float c1 = 1.0f - std::pow(b1, (float)M->step), c2 = 1.0f - std::pow(b2, (float)M->step);
-  hipLaunchKernelGGL(gg_mlp_adam, dim3((M->n_params + 255) / 256), dim3(256), 0, M->stream, M->p, M->g,
-                     M->m1, M->m2, M->n_params, lr, b1, b2, 1e-8f, c1, c2);
-  MLP_HIP(hipGetLastError());
-  if ((rc = refresh_bf16(M))) return rc;
-  MLP_HIP(hipStreamSynchronize(M->stream));
-  return GG_OK;
+  if (!M || !x || !t || m < 1) return set_error(GG_ERR_INVALID, "bad arguments");
+  return mlp_train(M, x, false, t, m, lr, loss);
 }
 
 }  // extern "C"
+
+namespace gg {
+
+int mlp_forward_device(gg_mlp *M, const float *x_dev, int m, float *y_host) {
+  if (!M || !x_dev || !y_host || m < 1) return set_error(GG_ERR_INVALID, "bad arguments");
+  int rc = mlp_forward(M, x_dev, m, true);
+  if (rc) return rc;
+  return copy_predictions(M, m, y_host);
+}
+
+int mlp_train_device(gg_mlp *M, const float *x_dev, const float *t_host, int m, float lr,
+                     float *loss) {
+  if (!M || !x_dev || !t_host || m < 1) return set_error(GG_ERR_INVALID, "bad arguments");
+  return mlp_train(M, x_dev, true, t_host, m, lr, loss);
+}
+
+}  // namespace gg
+

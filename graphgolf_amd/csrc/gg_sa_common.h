@@ -1,0 +1,87 @@
+// gg_sa_common.h -- small host helpers shared by the SA drivers and graph
+// builders (RNG, edge hash set, wall clock).
+#ifndef GG_SA_COMMON_H
+#define GG_SA_COMMON_H
+#include <chrono>
+#include <cstdint>
+#include <vector>
+
+namespace gg {
+
+// xoshiro256** seeded through splitmix64: the generators' own RNG (the SA
+// drivers use libc random() instead, to follow the reference's draws).
+struct Rng {
+  uint64_t s[4];
+  explicit Rng(uint64_t seed) {
+    for (int i = 0; i < 4; ++i) {
+      seed += 0x9E3779B97F4A7C15ull;
+      uint64_t z = seed;
+      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+      z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+      s[i] = z ^ (z >> 31);
+    }
+  }
+  static uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+  uint64_t next() {
+    const uint64_t r = rotl(s[1] * 5, 7) * 9;
+    const uint64_t t = s[1] << 17;
+    s[2] ^= s[0];
+    s[3] ^= s[1];
+    s[1] ^= s[2];
+    s[0] ^= s[3];
+    s[2] ^= t;
+    s[3] = rotl(s[3], 45);
+    return r;
+  }
+  // Uniform in [0, m) (Lemire's multiply-shift; bias < 2^-32 for our m).
+  uint32_t below(uint32_t m) { return (uint32_t)(((next() >> 32) * (uint64_t)m) >> 32); }
+};
+
+inline uint64_t ekey(int a, int b) {
+  return a < b ? ((uint64_t)(uint32_t)a << 32) | (uint32_t)b : ((uint64_t)(uint32_t)b << 32) | (uint32_t)a;
+}
+
+struct EdgeSet {  // open-addressing set of unordered edges
+  std::vector<uint64_t> slot;
+  uint64_t mask;
+  explicit EdgeSet(size_t n_edges) {
+    size_t cap = 16;
+    while (cap < n_edges * 4) cap <<= 1;
+    slot.assign(cap, ~0ull);
+    mask = cap - 1;
+  }
+  static uint64_t h(uint64_t k) {
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdull;
+    k ^= k >> 33;
+    return k;
+  }
+  bool has(uint64_t k) const {
+    for (uint64_t i = h(k) & mask;; i = (i + 1) & mask) {
+      if (slot[i] == k) return true;
+      if (slot[i] == ~0ull) return false;
+    }
+  }
+  void add(uint64_t k) {
+    uint64_t i = h(k) & mask;
+    while (slot[i] != ~0ull && slot[i] != ~1ull) i = (i + 1) & mask;
+    slot[i] = k;
+  }
+  void remove(uint64_t k) {
+    for (uint64_t i = h(k) & mask;; i = (i + 1) & mask) {
+      if (slot[i] == k) {
+        slot[i] = ~1ull;  // tombstone
+        return;
+      }
+      if (slot[i] == ~0ull) return;
+    }
+  }
+};
+
+inline double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+
+}  // namespace gg
+#endif

@@ -68,7 +68,8 @@ def sa_general(edges, nodes: int, degree: int, seed: int, ncalcs: int, max_temp:
 class BsaParams(ctypes.Structure):
     _fields_ = [("max_steps", ctypes.c_int64), ("time_budget_s", ctypes.c_double),
                 ("target_aspl", ctypes.c_double), ("proposals", ctypes.c_int32),
-                ("exact", ctypes.c_int32), ("explore", ctypes.c_int32),
+                ("exact", ctypes.c_int32), ("combine", ctypes.c_int32),
+                ("warmup", ctypes.c_int32), ("explore_every", ctypes.c_int32),
                 ("use_surrogate", ctypes.c_int32), ("max_temp", ctypes.c_double),
                 ("min_temp", ctypes.c_double), ("seed", ctypes.c_uint64),
                 ("train_steps", ctypes.c_int32), ("batch_train", ctypes.c_int32),
@@ -82,7 +83,8 @@ class BsaStats(ctypes.Structure):
                 ("pad", ctypes.c_int32), ("best_aspl", ctypes.c_double),
                 ("seconds_total", ctypes.c_double), ("seconds_exact", ctypes.c_double),
                 ("seconds_surrogate", ctypes.c_double), ("seconds_features", ctypes.c_double),
-                ("seconds_to_target", ctypes.c_double), ("surrogate_rank_corr", ctypes.c_double)]
+                ("seconds_to_target", ctypes.c_double), ("surrogate_rank_corr", ctypes.c_double),
+                ("moves_accepted", ctypes.c_int64), ("seconds_train", ctypes.c_double)]
 
 
 @dataclass
@@ -98,15 +100,15 @@ class BatchedResult:
 
 
 def sa_batched(edges, nodes: int, degree: int, *, max_steps: int = 100, time_budget_s: float = 0.0,
-               target_aspl: float = 0.0, proposals: int = 4096, exact: int = 256,
-               explore: int = 16, use_surrogate: bool = True, max_temp: float = 100.0,
-               min_temp: float = 0.217147, seed: int = 1, train_steps: int = 4,
-               batch_train: int = 512, lr: float = 1e-3, hist_maxd: int = 8,
-               feature_refresh: int = 1, log_capacity: int = 100000,
+               target_aspl: float = 0.0, proposals: int = 4096, exact: int = 1, combine: int = 4,
+               warmup: int = 256, explore_every: int = 8, use_surrogate: bool = True,
+               max_temp: float = 100.0, min_temp: float = 0.217147, seed: int = 1,
+               train_steps: int = 1, batch_train: int = 256, lr: float = 1e-3, hist_maxd: int = 8,
+               feature_refresh: int = 100, log_capacity: int = 100000,
                accepted_capacity: int = 100000) -> BatchedResult:
     """Batched (and, with use_surrogate, MLP-ranked) SA on the GPU; see
-    include/gg_host.h.  exact = proposals = 1 without the surrogate is plain
-    single-graph GPU SA with a device-resident adjacency."""
+    include/gg_host.h.  exact = 1 without the surrogate is plain single-graph
+    GPU SA with a device-resident adjacency."""
     lib = L.lib()
     if not getattr(lib, "_bsa_sigs", False):
         lib.gg_sa_batched.argtypes = [ctypes.c_int, ctypes.c_int, L.i32p,
@@ -120,9 +122,9 @@ def sa_batched(edges, nodes: int, degree: int, *, max_steps: int = 100, time_bud
         lib.gg_apply_move.restype = ctypes.c_int
         lib._bsa_sigs = True
     e = L.c32(edges).reshape(-1, 2).copy()
-    p = BsaParams(max_steps, time_budget_s, target_aspl, proposals, exact, explore,
-                  int(use_surrogate), max_temp, min_temp, seed, train_steps, batch_train, lr,
-                  hist_maxd, feature_refresh, log_capacity)
+    p = BsaParams(max_steps, time_budget_s, target_aspl, proposals, exact, combine, warmup,
+                  explore_every, int(use_surrogate), max_temp, min_temp, seed, train_steps,
+                  batch_train, lr, hist_maxd, feature_refresh, log_capacity)
     st = BsaStats()
     lt = np.zeros(max(log_capacity, 1))
     la = np.zeros(max(log_capacity, 1))
@@ -137,7 +139,7 @@ def sa_batched(edges, nodes: int, degree: int, *, max_steps: int = 100, time_bud
     L.check(rc)
     stats = {k: getattr(st, k) for k, _ in BsaStats._fields_ if k != "pad"}
     ns = min(int(st.steps), log_capacity)
-    na = min(int(st.accepts), accepted_capacity)
+    na = min(int(st.moves_accepted), accepted_capacity)
     return BatchedResult(e, stats, lt[:ns], la[:ns], ld[:ns], acc[:na], ad[:na], asum[:na])
 
 
@@ -145,3 +147,24 @@ def apply_move(edges: np.ndarray, line0: int, line1: int, variant: int) -> None:
     """Applies a logged batched-SA move to `edges` in place."""
     L.check(L.lib().gg_apply_move(edges.shape[0], L.ptr(edges), int(line0), int(line1),
                                   int(variant)))
+
+
+def sa_features(adj: np.ndarray, props: np.ndarray, vstat: np.ndarray | None = None) -> np.ndarray:
+    """The batched SA's device-computed move features ([count, 64] float32) for
+    moves props[i] = (a, b, c, d, r) of the graph `adj` ([n, d] int32)."""
+    lib = L.lib()
+    lib.gg_sa_features.argtypes = [ctypes.c_int, ctypes.c_int, L.i32p, ctypes.c_int, L.i32p,
+                                   ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
+    lib.gg_sa_features.restype = ctypes.c_int
+    adj = L.c32(adj)
+    props = L.c32(props).reshape(-1, 5)
+    n, d = adj.shape
+    out = np.zeros((props.shape[0], 64), dtype=np.float32)
+    fp = ctypes.POINTER(ctypes.c_float)
+    vs = None
+    if vstat is not None:
+        vstat = np.ascontiguousarray(vstat, dtype=np.float32).reshape(n, 10)
+        vs = vstat.ctypes.data_as(fp)
+    L.check(lib.gg_sa_features(n, d, L.ptr(adj), props.shape[0], L.ptr(props), vs,
+                               out.ctypes.data_as(fp)))
+    return out

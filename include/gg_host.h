@@ -81,31 +81,38 @@ int gg_sa_general(int nodes, int degree, int32_t *edges, const gg_sa_params *par
 /* ---- Batched, surrogate-ranked simulated annealing (no reference
  * counterpart; the accept rule is the reference's, general/sa.c:371-411).
  *
- * Each step draws `proposals` random 2-opt moves of the current graph; with
- * use_surrogate the MLP (include/gg_mlp.h) ranks them from per-vertex
- * distance-histogram features and the best `exact - explore` plus `explore`
- * random ones are scored exactly in one patched plan launch (without it,
- * `exact` random moves are scored).  The best scored move then faces the
- * reference's accept rule: smaller diameter wins; equal diameter accepts a
- * non-increasing S, else Metropolis exp(-dS/T).  Every exact (features, dS)
- * pair trains the MLP on the device.  exact = proposals = 1 without the
+ * Every step scores `exact` candidate graphs in one patched plan launch.
+ * A candidate is the current graph plus one or more vertex-disjoint 2-opt
+ * moves.  Explore steps (the first `warmup` steps, every `explore_every`-th
+ * step after them, and every step without the surrogate) score random single
+ * moves.  Ranked steps draw `proposals` random moves, compute their features
+ * on the device (per-vertex distance statistics of the four endpoints and
+ * 1-/2-hop ball overlaps of the removed and added pairs), rank them with the
+ * MLP (include/gg_mlp.h), and merge the best-ranked vertex-disjoint moves,
+ * up to `combine` per candidate, into each candidate.  The best candidate
+ * then faces the reference's accept rule: smaller diameter wins; equal
+ * diameter accepts a non-increasing S, else Metropolis exp(-dS/T).  Scored
+ * singles train the MLP on their exact dS; a scored combination trains its
+ * members on an equal share of its prediction error.  exact = 1 without the
  * surrogate is plain single-graph GPU SA on a device-resident adjacency.
- * groups = 1, no added centres. */
+ * Simple d-regular graphs, groups = 1, no added centres. */
 typedef struct gg_bsa_params {
   int64_t max_steps;
   double time_budget_s;   /* stop after this wall time (0: none) */
   double target_aspl;     /* stop once best ASPL <= target (0: none) */
-  int32_t proposals;
-  int32_t exact;
-  int32_t explore;
+  int32_t proposals;      /* moves ranked per ranked step */
+  int32_t exact;          /* candidate graphs scored per step */
+  int32_t combine;        /* max moves merged into one candidate (1..16) */
+  int32_t warmup;         /* initial explore steps */
+  int32_t explore_every;  /* every k-th later step explores (0: never) */
   int32_t use_surrogate;
   double max_temp, min_temp; /* geometric schedule over max_steps */
   uint64_t seed;
   int32_t train_steps;    /* Adam steps per SA step */
   int32_t batch_train;    /* rows per Adam step */
   float lr;
-  int32_t hist_maxd;      /* distance histogram depth for features (<= 15) */
-  int32_t feature_refresh;/* recompute features every k accepted moves */
+  int32_t hist_maxd;      /* distance histogram depth for the statistics (<= 15) */
+  int32_t feature_refresh;/* recompute per-vertex statistics every k accepts */
   int32_t log_capacity;   /* entries in the log arrays */
 } gg_bsa_params;
 
@@ -113,20 +120,34 @@ typedef struct gg_bsa_stats {
   int64_t steps, accepts, evaluations;
   int32_t best_diameter, pad;
   double best_aspl;
-  double seconds_total, seconds_exact, seconds_surrogate, seconds_features;
+  double seconds_total;
+  double seconds_exact;     /* candidate scoring: patch upload + plan replay */
+  double seconds_surrogate; /* feature kernel, MLP ranking and training */
+  double seconds_features;  /* host: proposal draws, candidate assembly, statistics refresh */
   double seconds_to_target; /* -1 when the target was not reached */
-  double surrogate_rank_corr; /* Spearman correlation, predicted vs exact, last 64 steps */
+  double surrogate_rank_corr; /* Spearman, prediction vs exact dS of explore singles (held out) */
+  int64_t moves_accepted;     /* 2-opt moves in accepted candidates */
+  double seconds_train;       /* MLP training (also counted in seconds_surrogate) */
 } gg_bsa_stats;
 
 /* log_time[k], log_best_aspl[k], log_best_diam[k]: after step k.
- * accepted[k*3 + {0,1,2}] = (line0, line1, variant) of the k-th accepted move,
- * accepted_diam[k] / accepted_sum[k] its exact diameter and S, so the run can
- * be replayed from the initial edges and every accepted graph re-checked.
+ * accepted[k*3 + {0,1,2}] = (line0, line1, variant) of the k-th accepted move
+ * in application order; accepted_diam[k] / accepted_sum[k] are the exact
+ * diameter and S of the graph after it when it ends an accepted candidate,
+ * and -1 / 0 for the earlier members of a merged candidate, so the run can be
+ * replayed from the initial edges and every accepted graph re-checked.
  * On return `edges` holds the best graph found. */
 int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa_params *params,
                   gg_bsa_stats *stats, double *log_time, double *log_best_aspl,
                   int32_t *log_best_diam, int32_t *accepted, int32_t *accepted_diam,
                   uint64_t *accepted_sum, int accepted_capacity);
+
+/* The surrogate's features of `count` moves props[i] = (a, b, c, d, r) of the
+ * graph adj (row-major nodes x degree), as gg_sa_batched computes them on the
+ * device: feats[i][0..63] (64 floats per move; layout in gg_bsa.hip).  vstat:
+ * per-vertex statistics [nodes][10], or NULL for zeros.  Exposed for tests. */
+int gg_sa_features(int nodes, int degree, const int32_t *adj, int count, const int32_t *props,
+                   const float *vstat, float *feats);
 
 /* Applies one logged move (line0, line1, variant) to an edge list the way
  * gg_sa_batched does (for replays). */

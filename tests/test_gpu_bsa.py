@@ -12,23 +12,36 @@ pytestmark = pytest.mark.gpu
 
 
 def replay_and_check(e0, n, d, res):
+    """Replays the logged moves in order; every move that ends an accepted
+    candidate (diameter >= 0 in the log) is re-scored by the oracle."""
     e = e0.copy()
+    checked = 0
     for k, (t0, t1, r) in enumerate(res.accepted):
         SA.apply_move(e, t0, t1, r)
+        if res.accepted_diam[k] < 0:
+            assert res.accepted_sum[k] == 0
+            continue
         o = oracle.matrix_op(G.edges_to_adj(n, d, e))
         assert o.ok and o.diameter == res.accepted_diam[k] and o.S == int(res.accepted_sum[k]), k
+        checked += 1
+    assert checked == res.stats["accepts"]
+    assert len(res.accepted) == res.stats["moves_accepted"]
     best = oracle.matrix_op(G.edges_to_adj(n, d, res.edges))
     assert best.diameter == res.stats["best_diameter"] and best.aspl == res.stats["best_aspl"]
     assert np.all(np.bincount(res.edges.ravel(), minlength=n) == d)
 
 
-@pytest.mark.parametrize("surrogate", [True, False])
-def test_gpu_batched_sa_every_accept_exact(surrogate):
-    n, d = 512, 4
+@pytest.mark.parametrize("surrogate,exact,combine", [(True, 16, 1), (True, 1, 4), (True, 4, 8),
+                                                     (False, 16, 1)])
+def test_gpu_batched_sa_every_accept_exact(surrogate, exact, combine):
+    n, d = 1024, 4
     e0 = G.random_regular(n, d, 3)
-    res = SA.sa_batched(e0, n, d, max_steps=25, proposals=256, exact=16, explore=4,
-                        use_surrogate=surrogate, seed=5, train_steps=2, batch_train=128)
+    res = SA.sa_batched(e0, n, d, max_steps=120, proposals=512, exact=exact, combine=combine,
+                        warmup=48, explore_every=4, use_surrogate=surrogate, seed=5,
+                        train_steps=2, batch_train=128, feature_refresh=5)
     assert res.stats["accepts"] > 0
+    if surrogate and combine > 1:
+        assert res.stats["moves_accepted"] > res.stats["accepts"]  # merged candidates were taken
     replay_and_check(e0, n, d, res)
 
 
@@ -38,3 +51,50 @@ def test_gpu_plain_device_sa():
     res = SA.sa_batched(e0, n, d, max_steps=300, proposals=1, exact=1, use_surrogate=False, seed=1)
     assert res.stats["evaluations"] == 301
     replay_and_check(e0, n, d, res)
+
+
+def _balls(adj, v):
+    b1 = {v} | set(adj[v].tolist())
+    b2 = set(b1)
+    for u in adj[v]:
+        b2 |= set(adj[u].tolist())
+    return b1, b2
+
+
+def features_reference(adj, props, vstat):
+    """Set-based restatement of the move features (float32 arithmetic)."""
+    n, d = adj.shape
+    f32 = np.float32
+    out = np.zeros((len(props), 64), dtype=np.float32)
+    for i, (a, b, c, e, r) in enumerate(props):
+        for k, v in enumerate((a, b, c, e)):
+            out[i, 10 * k:10 * k + 10] = vstat[v]
+        pairs = [(a, b), (c, e), (a, c if r else e), (b if r else c, e if r else b)]
+        for q, (x, y) in enumerate(pairs):
+            b1x, b2x = _balls(adj, x)
+            b1y, b2y = _balls(adj, y)
+            out[i, 40 + 4 * q] = f32(len(b1x & b1y)) / f32(d)
+            out[i, 41 + 4 * q] = f32(len(b1x & b2y)) / f32(d)
+            out[i, 42 + 4 * q] = f32(len(b2x & b2y)) / (f32(d) * f32(d))
+            out[i, 43 + 4 * q] = f32(0.5) * (vstat[x, 7] + vstat[y, 7])
+        out[i, 56] = r
+    return out
+
+
+@pytest.mark.parametrize("n,d", [(600, 6), (4096, 16), (65536, 16)])
+def test_gpu_move_features_match_set_reference(n, d):
+    rng = np.random.default_rng(n + d)
+    e = G.random_regular(n, d, 9)
+    adj = G.edges_to_adj(n, d, e)
+    props = []
+    while len(props) < 24:
+        t0, t1 = rng.integers(0, len(e), 2)
+        a, b = e[t0]
+        c, dd = e[t1]
+        if len({a, b, c, dd}) == 4:
+            props.append((a, b, c, dd, int(rng.integers(0, 2))))
+    props = np.array(props, dtype=np.int32)
+    vstat = rng.standard_normal((n, 10)).astype(np.float32)
+    got = SA.sa_features(adj, props, vstat)
+    want = features_reference(adj, props, vstat)
+    np.testing.assert_array_equal(got, want)
