@@ -12,6 +12,9 @@ reference's single-tile bit-parallel BFS,
     bytes = L * n * (4*d + 8*w*(d+2)),  w = ceil(B/64),  L = diameter,
 against MI355X's 8 TB/s HBM peak; the CPU baseline is the C/OpenMP port of the
 reference's matrix_op (oracle/) timed on this host on a bounded sample.
+For the default workload, `sa_time_to_aspl` adds the SA comparison: plain
+single-graph GPU SA for --sa-budget seconds, then surrogate-ranked SA until it
+reaches the same best ASPL (wall-clock ratio = `speedup`).
 """
 from __future__ import annotations
 
@@ -100,6 +103,45 @@ def cpu_baseline(items, budget_s):
                        f"{med:.4f} s; OpenMP threads={oracle.threads()}")
 
 
+def sa_time_to_aspl(budget_s):
+    """SA time-to-ASPL at n=65536 d=16 (random regular, seed 5): plain
+    single-graph GPU SA runs for budget_s; surrogate-ranked SA then runs until
+    its best ASPL reaches plain's final best.  Every accepted graph in both is
+    scored exactly by the evaluator."""
+    from graphgolf_amd import graphs as G
+    from graphgolf_amd import sa as SA
+
+    n, d = 65536, 16
+    e0 = G.random_regular(n, d, 5)
+    plain = SA.sa_batched(e0, n, d, max_steps=10**9, time_budget_s=budget_s,
+                          use_surrogate=False, exact=1, seed=1)
+    target = float(plain.log_best_aspl[-1])
+    t_plain = float(plain.log_time[int(np.argmax(plain.log_best_aspl <= target))])
+    sur = SA.sa_batched(e0, n, d, max_steps=10**9, time_budget_s=budget_s, target_aspl=target,
+                        proposals=4096, combine=16, warmup=200, explore_every=8, seed=1)
+    t_sur = float(sur.stats["seconds_to_target"])
+    return {"workload": "sa n=65536 d=16 (random regular start, seed 5)",
+            "target_aspl": target, "plain_seconds": round(t_plain, 3),
+            "plain_evaluations": int(plain.stats["evaluations"]),
+            "surrogate_seconds": round(t_sur, 3) if t_sur > 0 else None,
+            "surrogate_evaluations": int(sur.stats["evaluations"]),
+            "surrogate_moves_accepted": int(sur.stats["moves_accepted"]),
+            "surrogate_best_aspl": float(sur.stats["best_aspl"]),
+            "speedup": round(t_plain / t_sur, 2) if t_sur > 0 else None}
+
+
+def read_traffic(workload):
+    """HBM bytes per tile-BFS launch from the committed rocprofv3 PMC summary."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("hbm_bytes_per_launch:"):
+                return int(float(line.split(":", 1)[1]))
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,6 +151,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0,
                     help="seconds of CPU time for the CPU baseline sample (0 = skip)")
     ap.add_argument("--word-bits", type=int, default=0)
+    ap.add_argument("--sa-budget", type=float, default=20.0,
+                    help="seconds of plain GPU SA for the time-to-ASPL comparison (c5 only; 0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build evaluates on a single MI355X "
@@ -146,11 +190,7 @@ def main():
     k_ms = float(np.mean(kernel_ms))
     achieved = total_bytes / (k_ms * 1e-3) / 1e9
     r0 = res[0]
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
-    if os.path.exists(tf):
-        with open(tf) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+    traffic = read_traffic(args.workload)
     cpu = None
     if args.cpu_budget > 0:
         cpu = cpu_baseline(items, args.cpu_budget)
@@ -178,8 +218,10 @@ def main():
                      "algorithmic_bytes": int(total_bytes)},
         "cpu_baseline": cpu,
     }
-    print(json.dumps(line), flush=True)
     plan.close()
+    if args.workload == "c5" and args.sa_budget > 0:
+        line["sa_time_to_aspl"] = sa_time_to_aspl(args.sa_budget)
+    print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":

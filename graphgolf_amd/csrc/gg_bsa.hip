@@ -462,7 +462,7 @@ extern "C" int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa
 
   std::vector<Prop> props(npmax);
   std::vector<int32_t> props_h((size_t)npmax * 5);
-  std::vector<float> pred(npmax, 0.0f);
+  std::vector<float> pred(npmax, 0.0f), noise(npmax, 0.0f);
   std::vector<int> order(npmax);
   std::vector<Cand> cand(E);
   std::vector<int> cand_verts;
@@ -518,6 +518,9 @@ extern "C" int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa
       if ((rc = gg::mlp_forward_device(mlp.p, feats_d.p, np, pred.data()))) return rc;
       t_sur += now_s() - ts;
     }
+    std::vector<float> &rank = (use_sur && P->random_ranking && !explore) ? noise : pred;
+    if (&rank == &noise)
+      for (int i = 0; i < np; ++i) noise[i] = -1.0f - (float)rng.below(1u << 24) * (1.0f / (1u << 24));
     // --- candidates
     int ncand = explore ? E : 0;
     if (!explore) {
@@ -525,7 +528,7 @@ extern "C" int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa
       std::iota(order.begin(), order.begin() + np, 0);
       const int want = std::min(np, E * m_cur * 4 + 64);
       std::partial_sort(order.begin(), order.begin() + want, order.begin() + np,
-                        [&](int x, int y) { return pred[x] < pred[y]; });
+                        [&](int x, int y) { return rank[x] < rank[y]; });
       for (int c = 0; c < E; ++c) cand[c].nm = 0;
       // Greedy fill in rank order: candidate c takes up to m_cur valid moves
       // that share no vertex; a move predicted to raise S closes a non-empty
@@ -535,7 +538,7 @@ extern "C" int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa
       for (int oi = 0; oi < want && c < E; ++oi) {
         const int i = order[oi];
         const Prop &p = props[i];
-        if (cand[c].nm > 0 && pred[i] >= 0.0f) {
+        if (cand[c].nm > 0 && rank[i] >= 0.0f) {
           cand_verts.clear();
           if (++c >= E) break;
         }

@@ -70,7 +70,8 @@ class BsaParams(ctypes.Structure):
                 ("target_aspl", ctypes.c_double), ("proposals", ctypes.c_int32),
                 ("exact", ctypes.c_int32), ("combine", ctypes.c_int32),
                 ("warmup", ctypes.c_int32), ("explore_every", ctypes.c_int32),
-                ("use_surrogate", ctypes.c_int32), ("max_temp", ctypes.c_double),
+                ("use_surrogate", ctypes.c_int32), ("random_ranking", ctypes.c_int32),
+                ("max_temp", ctypes.c_double),
                 ("min_temp", ctypes.c_double), ("seed", ctypes.c_uint64),
                 ("train_steps", ctypes.c_int32), ("batch_train", ctypes.c_int32),
                 ("lr", ctypes.c_float), ("hist_maxd", ctypes.c_int32),
@@ -102,6 +103,7 @@ class BatchedResult:
 def sa_batched(edges, nodes: int, degree: int, *, max_steps: int = 100, time_budget_s: float = 0.0,
                target_aspl: float = 0.0, proposals: int = 4096, exact: int = 1, combine: int = 4,
                warmup: int = 256, explore_every: int = 8, use_surrogate: bool = True,
+               random_ranking: bool = False,
                max_temp: float = 100.0, min_temp: float = 0.217147, seed: int = 1,
                train_steps: int = 1, batch_train: int = 256, lr: float = 1e-3, hist_maxd: int = 8,
                feature_refresh: int = 100, log_capacity: int = 100000,
@@ -123,8 +125,8 @@ def sa_batched(edges, nodes: int, degree: int, *, max_steps: int = 100, time_bud
         lib._bsa_sigs = True
     e = L.c32(edges).reshape(-1, 2).copy()
     p = BsaParams(max_steps, time_budget_s, target_aspl, proposals, exact, combine, warmup,
-                  explore_every, int(use_surrogate), max_temp, min_temp, seed, train_steps,
-                  batch_train, lr, hist_maxd, feature_refresh, log_capacity)
+                  explore_every, int(use_surrogate), int(random_ranking), max_temp, min_temp,
+                  seed, train_steps, batch_train, lr, hist_maxd, feature_refresh, log_capacity)
     st = BsaStats()
     lt = np.zeros(max(log_capacity, 1))
     la = np.zeros(max(log_capacity, 1))

@@ -106,6 +106,8 @@ typedef struct gg_bsa_params {
   int32_t warmup;         /* initial explore steps */
   int32_t explore_every;  /* every k-th later step explores (0: never) */
   int32_t use_surrogate;
+  int32_t random_ranking; /* ablation: rank ranked-step proposals at random
+                             (the MLP still trains but does not choose) */
   double max_temp, min_temp; /* geometric schedule over max_steps */
   uint64_t seed;
   int32_t train_steps;    /* Adam steps per SA step */

@@ -5,6 +5,7 @@ import numpy as np
 import pytest
 
 import oracle
+from graphgolf_amd import evaluation as E
 from graphgolf_amd import graphs as G
 from graphgolf_amd import sa as SA
 
@@ -98,3 +99,45 @@ def test_gpu_move_features_match_set_reference(n, d):
     got = SA.sa_features(adj, props, vstat)
     want = features_reference(adj, props, vstat)
     np.testing.assert_array_equal(got, want)
+
+
+def _best_sum(n, d, res):
+    return E.apsp(G.edges_to_adj(n, d, res.edges)).sum
+
+
+def test_gpu_surrogate_gain_over_plain_and_random_ranking():
+    """Same number of SA steps (one exact evaluation each): MLP-ranked merged
+    candidates lower S more than plain SA and more than the same pipeline
+    with the ranking replaced by random scores."""
+    n, d, steps = 4096, 16, 500
+    e0 = G.random_regular(n, d, 21)
+    s0 = E.apsp(G.edges_to_adj(n, d, e0)).sum
+    kw = dict(max_steps=steps, exact=1, seed=2)
+    sur = dict(proposals=2048, combine=8, warmup=100, explore_every=8)
+    plain = SA.sa_batched(e0, n, d, use_surrogate=False, **kw)
+    mlp = SA.sa_batched(e0, n, d, **sur, **kw)
+    rnd = SA.sa_batched(e0, n, d, random_ranking=True, **sur, **kw)
+    gains = {k: s0 - _best_sum(n, d, r) for k, r in (("plain", plain), ("mlp", mlp), ("random", rnd))}
+    evals = {k: r.stats["evaluations"] for k, r in (("plain", plain), ("mlp", mlp), ("random", rnd))}
+    print("S gains", gains, "evaluations", evals, "rank corr", mlp.stats["surrogate_rank_corr"])
+    assert evals["mlp"] <= evals["plain"] and evals["random"] <= evals["plain"]
+    assert gains["mlp"] > 2 * gains["plain"], gains
+    assert gains["mlp"] > 2 * gains["random"], gains
+
+
+def test_gpu_surrogate_sa_full_size_accepts_are_exact():
+    """n = 65536, d = 16: every accepted candidate re-scored by the unpatched
+    evaluator after replaying the logged moves; the best graph by the oracle."""
+    n, d = 65536, 16
+    e0 = G.random_regular(n, d, 5)
+    res = SA.sa_batched(e0, n, d, max_steps=100, proposals=4096, combine=16, warmup=30,
+                        explore_every=8, seed=4)
+    assert res.stats["moves_accepted"] > res.stats["accepts"] > 0
+    e = e0.copy()
+    for k, (t0, t1, r) in enumerate(res.accepted):
+        SA.apply_move(e, t0, t1, r)
+        if res.accepted_diam[k] >= 0:
+            g = E.apsp(G.edges_to_adj(n, d, e))
+            assert (g.diameter, g.sum) == (res.accepted_diam[k], int(res.accepted_sum[k])), k
+    o = oracle.matrix_op(G.edges_to_adj(n, d, res.edges))
+    assert o.diameter == res.stats["best_diameter"] and o.aspl == res.stats["best_aspl"]
