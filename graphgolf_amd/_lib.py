@@ -55,6 +55,13 @@ class GGSaStats(ctypes.Structure):
 
 EVAL_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, i32p,
                            i32p, u64p, i32p)
+EVAL_GRID_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                ctypes.c_int, ctypes.c_int, i32p, i32p, i32p, u64p, i32p)
+
+
+class GGGridShape(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32),
+                ("groups", ctypes.c_int32), ("length", ctypes.c_int32)]
 
 
 class GGError(RuntimeError):
@@ -110,6 +117,13 @@ def lib():
         "gg_random_symmetric": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, i32p],
                                 ctypes.c_int),
         "gg_random_grid": ([ctypes.c_int] * 4 + [ctypes.c_uint64, i32p], ctypes.c_int),
+        "gg_random_grid_symmetric": ([ctypes.c_int] * 5 + [ctypes.c_uint64, i32p], ctypes.c_int),
+        "gg_grid_lower_bound": ([ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_int),
+                                                      ctypes.POINTER(ctypes.c_double)],
+                                ctypes.c_int),
+        "gg_sa_grid": ([ctypes.POINTER(GGGridShape), ctypes.c_int, i32p, i32p,
+                        ctypes.POINTER(GGSaParams), ctypes.c_double, EVAL_GRID_FN,
+                        ctypes.c_void_p, ctypes.POINTER(GGSaStats), i32p, u64p], ctypes.c_int),
         "gg_edges_to_adj": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, i32p, i32p], ctypes.c_int),
     }
     for name, (args, res) in sig.items():

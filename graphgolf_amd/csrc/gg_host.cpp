@@ -561,3 +561,403 @@ extern "C" int gg_sa_general(int nodes, int degree, int32_t *edges, const gg_sa_
   return GG_OK;
 }
 
+
+// ---------------------------------------------------------------------------
+// Grid SA (grid/sa.c:283-361): same RNG draws and moves as the reference.
+
+namespace {
+
+// Vertex v of a width x height grid sits at column v / height, row v % height
+// (WIDTH/HEIGHT, grid/common.c:53-61).
+struct GridGeom {
+  int height, width, groups;
+  int col(int v) const { return v / height; }
+  int row(int v) const { return v % height; }
+  int dist(int v, int w) const { return std::abs(col(v) - col(w)) + std::abs(row(v) - row(w)); }
+  // ROTATE (grid/common.c:63-86); groups == 4 assumes a square grid.
+  int rotate(int v, int deg) const {
+    const int w = col(v), h = row(v);
+    if (groups == 2) return (width - w - 1) * height + (height - h - 1);
+    if (deg == 90) return h * height + (height - w - 1);
+    if (deg == 180) return (height - w - 1) * height + (height - h - 1);
+    return (height - h - 1) * height + w;
+  }
+};
+
+bool same_edge(int a0, int a1, int b0, int b1) { return (a0 == b0 && a1 == b1) || (a0 == b1 && a1 == b0); }
+
+struct GridMover {
+  int lines, bl;  // lines, lines per group
+  GridGeom G;
+  int length;
+  // check_duplicate_current_edge (grid/common.c:262-307): lines of the base
+  // group (or, for a centre 1g move, every line outside the move's orbit)
+  // must not already hold one of the new edges.
+  bool fresh(const int32_t *e, int count, const int (*te)[2], int skip0, int skip1, bool all_lines,
+             int orbit) const {
+    const int end = all_lines ? lines : bl;
+    for (int i = 0; i < end; ++i) {
+      if (all_lines ? (i % bl == orbit) : (i == skip0 || i == skip1)) continue;
+      for (int j = 0; j < count; ++j)
+        if (same_edge(e[2 * i], e[2 * i + 1], te[j][0], te[j][1])) return false;
+    }
+    return true;
+  }
+  // edge_1g_opt (grid/sa.c:34-135).
+  bool one_group(int32_t *e, int start) const {
+    const int g = G.groups;
+    int tl[4], te[4][2], pattern;
+    for (int i = 0; i < g; ++i) {
+      const int t = start + bl * i;
+      tl[i] = t < lines ? t : t - lines;
+    }
+    if (e[2 * tl[0]] == e[2 * tl[g - 1] + 1] || e[2 * tl[0] + 1] == e[2 * tl[g - 1]]) return false;
+    const int s = e[2 * tl[0]], t = e[2 * tl[g / 2] + 1];
+    while (true) {
+      pattern = ref_random_below(g + 1);
+      if (g == 2) {
+        if (pattern != g) {
+          te[0][0] = s;
+          te[1][0] = G.rotate(s, 180);
+          te[0][1] = pattern == 0 ? t : G.rotate(t, 180);
+          te[1][1] = pattern == 0 ? G.rotate(t, 180) : t;
+        } else {
+          te[0][0] = s;
+          te[0][1] = G.rotate(s, 180);
+          te[1][0] = G.rotate(t, 180);
+          te[1][1] = t;
+        }
+      } else {  // g == 4
+        if (pattern != g) {
+          te[0][0] = s;
+          te[1][0] = G.rotate(s, 90);
+          te[2][0] = G.rotate(s, 180);
+          te[3][0] = G.rotate(s, 270);
+          // pattern k pairs rotation r of s with rotation (r - k) of t
+          const int rot[4] = {0, 90, 180, 270};
+          for (int i = 0; i < 4; ++i) {
+            const int k = ((i - pattern) % 4 + 4) % 4;
+            te[i][1] = k == 0 ? t : G.rotate(t, rot[k]);
+          }
+        } else {
+          te[0][0] = s;
+          te[0][1] = G.rotate(s, 180);
+          te[1][0] = G.rotate(s, 90);
+          te[1][1] = G.rotate(s, 270);
+          te[2][0] = t;
+          te[2][1] = G.rotate(t, 180);
+          te[3][0] = G.rotate(t, 90);
+          te[3][1] = G.rotate(t, 270);
+        }
+      }
+      if (t != te[g / 2][1]) break;
+    }
+    for (int i = 0; i < g; i += 2)
+      if (G.dist(te[i][0], te[i][1]) > length) return false;
+    if (!fresh(e, g, te, tl[0] % bl, -1, pattern == g, tl[0] % bl)) return false;
+    for (int i = 0; i < g; ++i) {
+      e[2 * tl[i]] = te[i][0];
+      e[2 * tl[i] + 1] = te[i][1];
+    }
+    return true;
+  }
+  // exchange_edge (grid/sa.c:137-221).
+  void exchange(int32_t *e) const {
+    const int g = G.groups;
+    int tl[8];
+    while (true) {
+      bool done = false, retry = false;
+      while (true) {
+        tl[0] = ref_random_below(lines);
+        tl[1] = ref_random_below(lines);
+        if (tl[0] == tl[1]) continue;
+        const int a = e[2 * tl[0]], b = e[2 * tl[0] + 1], c = e[2 * tl[1]], d = e[2 * tl[1] + 1];
+        if (a == c || b == d || a == d || b == c) continue;  // has_duplicated_vertex
+        if ((tl[0] - tl[1]) % bl == 0) {
+          if (one_group(e, tl[0])) {
+            done = true;
+            break;
+          }
+          continue;
+        }
+        break;
+      }
+      if (done) return;
+      auto corner_pair = [&](int t) {  // the edge joins a vertex and its 180-degree image
+        const int u = e[2 * t], v = e[2 * t + 1];
+        return G.col(u) + G.col(v) == G.width - 1 && G.row(u) + G.row(v) == G.height - 1;
+      };
+      if ((corner_pair(tl[0]) || corner_pair(tl[1])) && g % 2 == 0) {
+        if (one_group(e, tl[0])) return;
+        retry = true;
+      }
+      if (retry) continue;
+      for (int i = 1; i < g; ++i) {
+        tl[2 * i] = tl[0] + bl * i;
+        tl[2 * i + 1] = tl[1] + bl * i;
+        if (tl[2 * i] >= lines) tl[2 * i] -= lines;
+        if (tl[2 * i + 1] >= lines) tl[2 * i + 1] -= lines;
+      }
+      const int r = ref_random_below(2);
+      int te[8][2];
+      for (int i = 0; i < g; ++i) {
+        for (int j = 0; j < 2; ++j) {
+          te[2 * i][j] = e[2 * tl[2 * i] + j];
+          te[2 * i + 1][j] = e[2 * tl[2 * i + 1] + j];
+        }
+        std::swap(te[2 * i][1], te[2 * i + 1][r]);
+      }
+      if (G.dist(te[0][0], te[0][1]) > length || G.dist(te[1][0], te[1][1]) > length) continue;
+      bool ok = true;  // check_duplicate_tmp_edge (grid/common.c:246-260)
+      for (int i = 0; i < 2; ++i)
+        for (int j = 2; j < 2 * g; ++j)
+          if (same_edge(te[i][0], te[i][1], te[j][0], te[j][1])) ok = false;
+      if (!ok) continue;
+      if (!fresh(e, 2 * g, te, tl[0] % bl, tl[1] % bl, false, -1)) continue;
+      for (int i = 0; i < g; ++i)
+        for (int j = 0; j < 2; ++j) {
+          e[2 * tl[2 * i] + j] = te[2 * i][j];
+          e[2 * tl[2 * i + 1] + j] = te[2 * i + 1][j];
+        }
+      return;
+    }
+  }
+};
+
+int device_eval_grid(void *, int nodes, int degree, int groups, int based_nodes, const int32_t *adj,
+                     const int32_t *rotate_hash, int32_t *diameter, uint64_t *sum, int32_t *connected) {
+  gg_graph g = {nodes, degree, based_nodes, groups, 0, GG_GRAPH_SYMMETRIC, adj, rotate_hash};
+  gg_result r;
+  int rc = gg_apsp(&g, &r);
+  if (rc) return rc;
+  *diameter = r.diameter;
+  *sum = r.sum;
+  *connected = r.connected;
+  return GG_OK;
+}
+
+}  // namespace
+
+// lower_bound_of_diam_aspl (grid/main.c:480-529): per vertex, the number of
+// vertices within k hops is at most min(Moore count, grid vertices within
+// Manhattan distance k * length).  The diameter bound is read from the last
+// vertex's profile, as the reference does.
+extern "C" int gg_grid_lower_bound(int width, int height, int degree, int length, int *low_diam,
+                                   double *low_aspl) {
+  if (width < 1 || height < 1 || degree < 2 || length < 1 || !low_diam || !low_aspl)
+    return set_error(GG_ERR_INVALID, "bad grid lower-bound arguments");
+  const int m = width, n = height, mn = m * n;
+  std::vector<long long> moore(2, 0);
+  moore[0] = 1;
+  moore[1] = degree + 1;
+  long long current = degree;
+  int ii;
+  for (ii = 2;; ++ii) {
+    current *= degree - 1;
+    moore.push_back(moore[ii - 1] + current);
+    if (moore[ii] >= mn) {
+      moore[ii] = mn;
+      break;
+    }
+  }
+  const int maxhop = std::max((m + n - 2 + (length - 1)) / length, ii);
+  moore.resize(maxhop + 1, mn);
+  for (int i = ii + 1; i <= maxhop; ++i) moore[i] = mn;
+  std::vector<long long> hist(maxhop + 1), mh(maxhop + 1);
+  double sum = 0;
+  for (int i = 0; i < m; ++i)
+    for (int j = 0; j < n; ++j) {
+      std::fill(hist.begin(), hist.end(), 0);
+      for (int i2 = 0; i2 < m; ++i2)
+        for (int j2 = 0; j2 < n; ++j2) hist[(std::abs(i - i2) + std::abs(j - j2) + length - 1) / length]++;
+      for (int k = 1; k <= maxhop; ++k) hist[k] += hist[k - 1];
+      for (int k = 0; k <= maxhop; ++k) mh[k] = std::min(hist[k], moore[k]);
+      for (int k = 1; k <= maxhop; ++k) sum += (double)(mh[k] - mh[k - 1]) * k;
+    }
+  int dboth = 0;
+  while (mh[dboth] != mn) ++dboth;
+  *low_diam = dboth;
+  *low_aspl = sum / ((double)mn * (mn - 1));
+  return GG_OK;
+}
+
+extern "C" int gg_sa_grid(const gg_grid_shape *shape, int degree, int32_t *edges,
+                          const int32_t *rotate_hash, const gg_sa_params *P, double fixed_temp,
+                          gg_eval_grid_fn eval, void *ctx, gg_sa_stats *stats, int32_t *log_diam,
+                          uint64_t *log_sum) {
+  if (!shape || !edges || !P || degree < 1)
+    return set_error(GG_ERR_INVALID, "bad grid SA arguments");
+  const int width = shape->width, height = shape->height, groups = shape->groups;
+  if (width < 1 || height < 1 || (groups != 1 && groups != 2 && groups != 4) || shape->length < 1)
+    return set_error(GG_ERR_INVALID, "bad grid shape %dx%d g=%d r=%d", width, height, groups,
+                     shape->length);
+  if ((groups == 4 && width != height) || (groups == 2 && height % 2))
+    return set_error(GG_ERR_INVALID, "grid %dx%d cannot carry %d-fold symmetry", width, height, groups);
+  const int nodes = width * height;
+  if (((int64_t)nodes * degree) % 2 || nodes % groups || (nodes * degree / 2) % groups)
+    return set_error(GG_ERR_INVALID, "nodes*degree/2 must be a whole multiple of groups");
+  const int lines = nodes * degree / 2;
+  std::vector<int32_t> ident;
+  if (!rotate_hash) {
+    if (groups != 1) return set_error(GG_ERR_INVALID, "groups > 1 needs rotate_hash");
+    ident.resize(nodes);
+    std::iota(ident.begin(), ident.end(), 0);
+    rotate_hash = ident.data();
+  }
+  for (int i = 0; i < 2 * lines; ++i)
+    if (edges[i] < 0 || edges[i] >= nodes) return set_error(GG_ERR_INVALID, "edge endpoint out of range");
+  if (!eval) eval = device_eval_grid;
+  const double t_start = now_s();
+  double t_eval = 0;
+  GridMover M{lines, lines / groups, GridGeom{height, width, groups}, shape->length};
+  std::vector<int32_t> tmp(edges, edges + 2 * (size_t)lines), best(tmp), adj((size_t)nodes * degree);
+  int64_t logn = 0, evals = 0;
+  auto evaluate = [&](const int32_t *e, int *diam, double *aspl) -> int {
+    int rc = gg_edges_to_adj(nodes, lines, degree, e, adj.data());
+    if (rc) return rc;
+    int32_t dm = 0, conn = 0;
+    uint64_t sum = 0;
+    const double t0 = now_s();
+    rc = eval(ctx, nodes, degree, groups, nodes / groups, adj.data(), rotate_hash, &dm, &sum, &conn);
+    t_eval += now_s() - t0;
+    if (rc) return rc;
+    ++evals;
+    if (logn < P->log_capacity) {
+      if (log_diam) log_diam[logn] = dm;
+      if (log_sum) log_sum[logn] = sum;
+      ++logn;
+    }
+    const bool ok = conn && !(dm > nodes);  // grid/evaluation.c:182-185
+    *diam = dm;
+    *aspl = ok ? (double)sum / (((double)nodes - 1) * nodes) : 0.0;
+    return ok ? 1 : 0;
+  };
+  int low_diam;
+  double low_aspl;
+  int rc = gg_grid_lower_bound(width, height, degree, shape->length, &low_diam, &low_aspl);
+  if (rc) return rc;
+  const bool fixed = fixed_temp > 0 && !P->hill_climbing;
+  const double max_t = P->hill_climbing ? 0.0 : P->max_temp;
+  const double min_t = P->hill_climbing ? 0.0 : P->min_temp;
+  const int cycle = P->cooling_cycle > 0 ? P->cooling_cycle : 1;
+  const double rate = P->hill_climbing ? 1.0 : std::pow(min_t / max_t, (double)cycle / P->ncalcs);
+  double temp = max_t;
+  srandom(P->seed);
+  int cur_diam, best_diam;
+  double cur_aspl, best_aspl;
+  rc = evaluate(edges, &cur_diam, &cur_aspl);
+  if (rc < 0) return rc;
+  if (rc == 0) return set_error(GG_ERR_INVALID, "the start graph is not connected");
+  best_diam = cur_diam;
+  best_aspl = cur_aspl;
+  int64_t accepts = 0, ii;
+  for (ii = 0; ii < P->ncalcs; ++ii) {
+    int nd;
+    double na;
+    while (true) {
+      std::copy(edges, edges + 2 * (size_t)lines, tmp.begin());
+      M.exchange(tmp.data());
+      rc = evaluate(tmp.data(), &nd, &na);
+      if (rc < 0) return rc;
+      if (rc == 1) break;
+    }
+    // accept (grid/sa.c:223-268)
+    const double diff = ((cur_aspl - na) * nodes * (nodes - 1)) / groups;
+    bool acc;
+    if (nd < cur_diam) acc = true;
+    else if (nd > cur_diam) acc = false;
+    else if (diff >= 0) acc = true;
+    else if (P->hill_climbing) acc = false;
+    else acc = std::exp(diff / (fixed ? fixed_temp : temp)) > ref_uniform();
+    if (acc) {
+      ++accepts;
+      cur_aspl = na;
+      cur_diam = nd;
+      std::copy(tmp.begin(), tmp.end(), edges);
+      if (best_diam > nd || (best_diam == nd && best_aspl > na)) {
+        std::copy(tmp.begin(), tmp.end(), best.begin());
+        best_diam = nd;
+        best_aspl = na;
+      }
+      if (best_diam == low_diam && best_aspl == low_aspl) break;
+    }
+    if ((ii + 1) % cycle == 0) temp *= rate;
+  }
+  std::copy(best.begin(), best.end(), edges);
+  if (stats) {
+    stats->steps = ii;
+    stats->accepts = accepts;
+    stats->evaluations = evals;
+    stats->best_diameter = best_diam;
+    stats->best_aspl = best_aspl;
+    stats->seconds_total = now_s() - t_start;
+    stats->seconds_eval = t_eval;
+  }
+  return GG_OK;
+}
+
+namespace {
+// Vertices not reached from vertex 0 (simple_bfs, grid/main.c:197-230).
+int unreached_from_zero(int n, int d, const std::vector<int32_t> &adj) {
+  std::vector<char> seen(n, 0);
+  std::vector<int> q{0};
+  seen[0] = 1;
+  for (size_t h = 0; h < q.size(); ++h)
+    for (int j = 0; j < d; ++j) {
+      const int u = adj[(size_t)q[h] * d + j];
+      if (!seen[u]) {
+        seen[u] = 1;
+        q.push_back(u);
+      }
+    }
+  return n - (int)q.size();
+}
+}  // namespace
+
+// create_symmetric_edge (grid/main.c:393-445) over a seeded base lattice.
+extern "C" int gg_random_grid_symmetric(int width, int height, int degree, int length, int groups,
+                                        uint64_t seed, int32_t *edges) {
+  if (!edges || (groups != 1 && groups != 2 && groups != 4))
+    return set_error(GG_ERR_INVALID, "groups must be 1, 2 or 4");
+  if (groups == 1) return gg_random_grid(width, height, degree, length, seed, edges);
+  if ((groups == 4 && (width != height || width % 2)) || height % 2)
+    return set_error(GG_ERR_INVALID, "grid %dx%d cannot carry %d-fold symmetry", width, height, groups);
+  const int bw = groups == 4 ? width / 2 : width, bh = height / 2;
+  const int nodes = width * height, lines = nodes * degree / 2, bl = lines / groups;
+  if (((int64_t)nodes * degree) % 2 || lines % groups)
+    return set_error(GG_ERR_INVALID, "nodes*degree/2 must be a whole multiple of groups");
+  std::vector<int32_t> base((size_t)2 * bl);
+  int rc = gg_random_grid(bw, bh, degree, length, seed, base.data());
+  if (rc) return rc;
+  GridGeom G{height, width, groups};
+  for (int i = 0; i < 2 * bl; ++i) edges[i] = (base[i] / bh) * height + base[i] % bh;
+  for (int i = 0; i < bl; ++i)
+    for (int j = 0; j < 2; ++j) {
+      const int v = edges[2 * i + j];
+      if (groups == 2) {
+        edges[2 * (bl + i) + j] = G.rotate(v, 180);
+      } else {
+        edges[2 * (bl + i) + j] = G.rotate(v, 90);
+        edges[2 * (2 * bl + i) + j] = G.rotate(v, 180);
+        edges[2 * (3 * bl + i) + j] = G.rotate(v, 270);
+      }
+    }
+  GridMover M{lines, bl, G, length};
+  srandom((unsigned)seed);
+  for (int64_t i = 0; i < (int64_t)lines * 10; ++i) M.exchange(edges);  // INITIAL_TIMES
+  std::vector<int32_t> adj((size_t)nodes * degree), tmp(edges, edges + 2 * (size_t)lines);
+  if ((rc = gg_edges_to_adj(nodes, lines, degree, edges, adj.data()))) return rc;
+  int min_num = unreached_from_zero(nodes, degree, adj);
+  for (int64_t tries = 0; min_num != 0; ++tries) {
+    if (tries > 1000000) return set_error(GG_ERR_INVALID, "could not connect the symmetric grid graph");
+    std::copy(edges, edges + 2 * (size_t)lines, tmp.begin());
+    M.exchange(tmp.data());
+    if ((rc = gg_edges_to_adj(nodes, lines, degree, tmp.data(), adj.data()))) return rc;
+    const int num = unreached_from_zero(nodes, degree, adj);
+    if (num <= min_num) {
+      min_num = num;
+      std::copy(tmp.begin(), tmp.end(), edges);
+    }
+  }
+  return GG_OK;
+}

@@ -6,6 +6,8 @@ C++), so the same seed gives the same graph everywhere.
 """
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 
 from . import _lib as L
@@ -32,6 +34,26 @@ def random_grid(width: int, height: int, degree: int, length: int, seed: int) ->
     e = np.zeros((n * degree // 2, 2), dtype=np.int32)
     L.check(L.lib().gg_random_grid(width, height, degree, length, seed, L.ptr(e)))
     return e
+
+
+def random_grid_symmetric(width: int, height: int, degree: int, length: int, groups: int,
+                          seed: int) -> np.ndarray:
+    """groups-fold rotationally symmetric grid graph (create_symmetric_edge,
+    grid/main.c:393-445) grown from a seeded base lattice."""
+    n = width * height
+    e = np.zeros((n * degree // 2, 2), dtype=np.int32)
+    L.check(L.lib().gg_random_grid_symmetric(width, height, degree, length, groups, seed,
+                                             L.ptr(e)))
+    return e
+
+
+def grid_lower_bound(width: int, height: int, degree: int, length: int):
+    """(diameter, ASPL) lower bounds of a grid graph (grid/main.c:480-529)."""
+    d = ctypes.c_int(0)
+    a = ctypes.c_double(0)
+    L.check(L.lib().gg_grid_lower_bound(width, height, degree, length, ctypes.byref(d),
+                                        ctypes.byref(a)))
+    return d.value, a.value
 
 
 def edges_to_adj(nodes: int, degree: int, edges) -> np.ndarray:

@@ -65,6 +65,49 @@ def sa_general(edges, nodes: int, degree: int, seed: int, ncalcs: int, max_temp:
                     ld[:n_log] if log else None, ls[:n_log] if log else None)
 
 
+def sa_grid(edges, width: int, height: int, degree: int, length: int, seed: int, ncalcs: int,
+            groups: int = 1, max_temp: float = 100.0, min_temp: float = 0.217147,
+            cooling_cycle: int = 1, hill_climbing: bool = False, fixed_temp: float = 0.0,
+            evaluator: Optional[Callable] = None, log: int = 0) -> SaResult:
+    """Plain SA on a grid graph (grid/sa.c:283-361).  `evaluator(adj, groups,
+    based_nodes, rotate_hash) -> (connected, diameter, S)` replaces the device
+    evaluator when given."""
+    from .graphs import rotate_hash as make_rotate_hash
+
+    nodes = width * height
+    e = L.c32(edges).reshape(-1, 2).copy()
+    rh = make_rotate_hash(nodes, height, width, groups)
+    shape = L.GGGridShape(width, height, groups, length)
+    p = L.GGSaParams(ncalcs, max_temp, min_temp, cooling_cycle, int(hill_climbing), seed, log)
+    st = L.GGSaStats()
+    ld = np.zeros(max(log, 1), dtype=np.int32)
+    ls = np.zeros(max(log, 1), dtype=np.uint64)
+    cb = ctypes.cast(None, L.EVAL_GRID_FN)
+    keep = None
+    if evaluator is not None:
+        def _cb(ctx, n, d, g, b, adj_p, rh_p, diam_p, sum_p, conn_p):
+            try:
+                adj = np.ctypeslib.as_array(adj_p, shape=(n, d)).copy()
+                rhash = np.ctypeslib.as_array(rh_p, shape=(n,)).copy()
+                ok, dm, S = evaluator(adj, g, b, rhash)
+                diam_p[0] = int(dm)
+                sum_p[0] = int(S)
+                conn_p[0] = int(bool(ok))
+                return 0
+            except Exception:  # surfaced as an error code
+                return -1
+        keep = L.EVAL_GRID_FN(_cb)
+        cb = keep
+    rc = L.lib().gg_sa_grid(ctypes.byref(shape), degree, L.ptr(e), L.ptr(rh), ctypes.byref(p),
+                            fixed_temp, cb, None, ctypes.byref(st), L.ptr(ld),
+                            ls.ctypes.data_as(L.u64p))
+    L.check(rc)
+    n_log = min(log, st.evaluations)
+    return SaResult(e, st.steps, st.accepts, st.evaluations, st.best_diameter, st.best_aspl,
+                    st.seconds_total, st.seconds_eval,
+                    ld[:n_log] if log else None, ls[:n_log] if log else None)
+
+
 class BsaParams(ctypes.Structure):
     _fields_ = [("max_steps", ctypes.c_int64), ("time_budget_s", ctypes.c_double),
                 ("target_aspl", ctypes.c_double), ("proposals", ctypes.c_int32),

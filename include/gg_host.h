@@ -78,6 +78,41 @@ int gg_sa_general(int nodes, int degree, int32_t *edges, const gg_sa_params *par
                   gg_eval_fn eval, void *eval_ctx, gg_sa_stats *stats, int32_t *log_diam,
                   uint64_t *log_sum);
 
+/* ---- Grid SA (grid/sa.c:283-361), same RNG draws (libc random() after
+ * srandom(seed)) and moves as the reference: length-limited 2-opt with
+ * groups-fold rotational symmetry (exchange_edge, grid/sa.c:137-221;
+ * edge_1g_opt, :34-135), disconnected proposals redrawn, accept
+ * grid/sa.c:223-268, stop at the lower bound (grid/main.c:480-529). */
+typedef struct gg_grid_shape {
+  int32_t width, height; /* nodes = width * height, vertex v at (v / height, v % height) */
+  int32_t groups;        /* 1, 2 or 4 */
+  int32_t length;        /* max Manhattan length of an edge (-r) */
+} gg_grid_shape;
+
+/* Grid evaluator callback: like gg_eval_fn plus the symmetry inputs. */
+typedef int (*gg_eval_grid_fn)(void *ctx, int nodes, int degree, int groups, int based_nodes,
+                               const int32_t *adj, const int32_t *rotate_hash, int32_t *diameter,
+                               uint64_t *sum, int32_t *connected);
+
+/* A groups-fold symmetric grid graph (create_symmetric_edge,
+ * grid/main.c:393-445): a seeded base lattice (as gg_random_grid) on the
+ * base sub-grid, its rotated copies, lines*10 symmetric exchanges, then
+ * exchanges until connected (libc random() after srandom(seed)). */
+int gg_random_grid_symmetric(int width, int height, int degree, int length, int groups,
+                             uint64_t seed, int32_t *edges);
+
+/* lower_bound_of_diam_aspl (grid/main.c:480-529). */
+int gg_grid_lower_bound(int width, int height, int degree, int length, int *low_diam,
+                        double *low_aspl);
+
+/* params: as for gg_sa_general; fixed_temp > 0 uses that temperature in the
+ * Metropolis test (-F).  rotate_hash: create_rotate_hash's permutation
+ * (grid/common.c:312-341), NULL for groups = 1.  eval NULL = device
+ * evaluator.  On return `edges` holds the best graph. */
+int gg_sa_grid(const gg_grid_shape *shape, int degree, int32_t *edges, const int32_t *rotate_hash,
+               const gg_sa_params *params, double fixed_temp, gg_eval_grid_fn eval, void *eval_ctx,
+               gg_sa_stats *stats, int32_t *log_diam, uint64_t *log_sum);
+
 /* ---- Batched, surrogate-ranked simulated annealing (no reference
  * counterpart; the accept rule is the reference's, general/sa.c:371-411).
  *

@@ -278,3 +278,79 @@ def ref_sa_general(edges, nodes, degree, seed, ncalcs, max_temp=100.0, min_temp=
                  bool(hill_climbing), False, ctypes.byref(mde), _ip(e), ctypes.byref(diam),
                  ctypes.byref(aspl), cooling_cycle, 0, nodes, ctypes.byref(tot), True, 1)
     return e, diam.value, aspl.value, int(steps)
+
+
+def grid_lower_bound(width, height, degree, length):
+    """Restatement of lower_bound_of_diam_aspl (grid/main.c:480-529): per
+    vertex, min(Moore count, grid vertices within k*length Manhattan steps)
+    vertices lie within k hops; the diameter bound is read from the last
+    vertex's profile as the reference does."""
+    m, n = width, height
+    mn = m * n
+    moore = [1, degree + 1]
+    current = degree
+    ii = 2
+    while True:
+        current *= degree - 1
+        moore.append(moore[ii - 1] + current)
+        if moore[ii] >= mn:
+            moore[ii] = mn
+            break
+        ii += 1
+    maxhop = max((m + n - 2 + (length - 1)) // length, ii)
+    moore = np.array((moore + [mn] * (maxhop + 1))[:maxhop + 1], dtype=np.int64)
+    cols, rows = np.meshgrid(np.arange(m), np.arange(n), indexing="ij")
+    cols, rows = cols.ravel(), rows.ravel()
+    total = 0.0
+    mh = None
+    for i in range(m):
+        for j in range(n):
+            hop = (np.abs(cols - i) + np.abs(rows - j) + length - 1) // length
+            hist = np.cumsum(np.bincount(hop, minlength=maxhop + 1)[:maxhop + 1])
+            mh = np.minimum(hist, moore)
+            for k in range(1, maxhop + 1):
+                total += float(mh[k] - mh[k - 1]) * k
+    dboth = int(np.argmax(mh == mn))
+    return dboth, total / (float(mn) * (mn - 1))
+
+
+def ref_sa_grid(edges, width, height, degree, length, groups, seed, ncalcs, max_temp=100.0,
+                min_temp=0.217147, cooling_cycle=1, hill_climbing=False, fixed_temp=0.0):
+    """Run the reference's grid sa() (grid/sa.c:283-361) from srandom(seed) on
+    a copy of `edges`.  Returns (best_edges, best_diam, best_aspl, steps)."""
+    import math
+
+    L = ref_lib("grid")
+    e = np.ascontiguousarray(edges, dtype=np.int32).copy()
+    lines = e.shape[0]
+    nodes = width * height
+    low_diam, low_aspl = grid_lower_bound(width, height, degree, length)
+    if hill_climbing:
+        fixed_temp = max_temp = min_temp = 0.0
+        rate = 1.0
+    else:
+        rate = math.pow(min_temp / max_temp, cooling_cycle / ncalcs)
+    bw = width // 2 if groups == 4 else width
+    bh = height // 2 if groups > 1 else height
+    rh = ref_rotate_hash(nodes, height, width, groups)
+    ip = ctypes.POINTER(ctypes.c_int)
+    L.sa.restype = ctypes.c_longlong
+    L.sa.argtypes = ([ctypes.c_int] * 4 + [ctypes.c_longlong, ctypes.c_double, ctypes.c_int,
+                                            ctypes.c_double, ctypes.c_bool, ctypes.c_bool,
+                                            ctypes.c_bool, ctypes.POINTER(ctypes.c_double),
+                                            ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                            ip, ip, ctypes.POINTER(ctypes.c_double), ctypes.c_int,
+                                            ctypes.POINTER(ctypes.c_longlong)]
+                     + [ctypes.c_int] * 6 + [ip, ctypes.c_bool])
+    L.ref_srandom.argtypes = [ctypes.c_uint]
+    mde = ctypes.c_double(0)
+    diam = ctypes.c_int(0)
+    aspl = ctypes.c_double(0)
+    tot = ctypes.c_longlong(0)
+    L.ref_srandom(seed)
+    steps = L.sa(nodes, lines, degree, nodes // groups, ncalcs, rate, low_diam, low_aspl, False,
+                 bool(hill_climbing), False, ctypes.byref(mde), max_temp, min_temp, fixed_temp,
+                 _ip(e), ctypes.byref(diam), ctypes.byref(aspl), cooling_cycle,
+                 ctypes.byref(tot), width, bw, height, bh, length, groups, _ip(rh),
+                 bool(fixed_temp > 0 and not hill_climbing))
+    return e, diam.value, aspl.value, int(steps)

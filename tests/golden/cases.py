@@ -15,6 +15,7 @@ from graphgolf_amd import graphs as G
 DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
 GOLDEN_CSV = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden_eval.csv")
 GOLDEN_SA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden_sa.json")
+GOLDEN_GRID_SA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden_grid_sa.json")
 
 
 def two_opt(edges: np.ndarray, nodes: int, rng: np.random.Generator, moves: int = 1) -> np.ndarray:
@@ -162,4 +163,26 @@ def sa_cases():
         dict(id="sa:n512d4", n=512, d=4, graph_seed=3, seed=1, ncalcs=3000),
         dict(id="sa:hill", n=128, d=6, graph_seed=4, seed=5, ncalcs=2000, hill=True),
         dict(id="sa:cycle5", n=200, d=5, graph_seed=5, seed=9, ncalcs=3000, cooling_cycle=5),
+    ]
+
+
+def grid_sa_cases():
+    """Grid SA runs pinned against the reference's grid sa(): the C4 shape
+    (100x100, d=4, r=2) and smaller grids with 2- and 4-fold symmetry (these
+    exercise the 1g moves), hill climbing, a cooling cycle and -F."""
+    return [
+        dict(id="gsa:c4", width=100, height=100, degree=4, length=2, groups=1, graph_seed=1,
+             seed=3, ncalcs=200),
+        dict(id="gsa:w10h10d4r2", width=10, height=10, degree=4, length=2, groups=1,
+             graph_seed=2, seed=5, ncalcs=2000),
+        dict(id="gsa:w10h10d4r3g2", width=10, height=10, degree=4, length=3, groups=2,
+             graph_seed=3, seed=7, ncalcs=2000),
+        dict(id="gsa:w10h10d4r3g4", width=10, height=10, degree=4, length=3, groups=4,
+             graph_seed=4, seed=9, ncalcs=2000),
+        dict(id="gsa:w8h8d6r3g4:hill", width=8, height=8, degree=6, length=3, groups=4,
+             graph_seed=5, seed=11, ncalcs=1000, hill=True),
+        dict(id="gsa:w12h6d4r2g2:cycle", width=12, height=6, degree=4, length=2, groups=2,
+             graph_seed=6, seed=13, ncalcs=1500, cooling_cycle=3),
+        dict(id="gsa:w10h10d6r3:fixed", width=10, height=10, degree=6, length=3, groups=1,
+             graph_seed=7, seed=15, ncalcs=1000, fixed_temp=20.0),
     ]

@@ -27,13 +27,35 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
 
 import oracle  # noqa: E402
-from cases import GOLDEN_CSV, GOLDEN_SA, build, general_cases, grid_cases, sa_cases  # noqa: E402
+from cases import (GOLDEN_CSV, GOLDEN_GRID_SA, GOLDEN_SA, build, general_cases,  # noqa: E402
+                   grid_cases, grid_sa_cases, sa_cases)
 from graphgolf_amd import graphs as G  # noqa: E402
+
+
+def grid_sa() -> None:
+    """Outcomes of the reference's grid sa() (grid/sa.c:283-361)."""
+    rows = []
+    for case in grid_sa_cases():
+        w, h, d, r, g = case["width"], case["height"], case["degree"], case["length"], case["groups"]
+        e0 = G.random_grid_symmetric(w, h, d, r, g, case["graph_seed"])
+        e, diam, aspl, steps = oracle.ref_sa_grid(
+            e0, w, h, d, r, g, case["seed"], case["ncalcs"],
+            cooling_cycle=case.get("cooling_cycle", 1), hill_climbing=case.get("hill", False),
+            fixed_temp=case.get("fixed_temp", 0.0))
+        rows.append(dict(case, start_sha256=hashlib.sha256(e0.tobytes()).hexdigest(),
+                         best_diameter=diam, best_aspl_hex=float(aspl).hex(), steps=steps,
+                         edges_sha256=hashlib.sha256(e.tobytes()).hexdigest()))
+        print(rows[-1], flush=True)
+    with open(GOLDEN_GRID_SA, "w") as f:
+        json.dump(rows, f, indent=1)
 
 
 def main() -> None:
     if not (oracle.ref_available("general") and oracle.ref_available("grid")):
         raise SystemExit("oracle/_ref is not built (make -C oracle ref)")
+    if "--grid-sa" in sys.argv:
+        grid_sa()
+        return
     rows = []
     for case in general_cases():
         adj, kw = build(case)
@@ -64,6 +86,7 @@ def main() -> None:
         print(sa_rows[-1], flush=True)
     with open(GOLDEN_SA, "w") as f:
         json.dump(sa_rows, f, indent=1)
+    grid_sa()
 
 
 if __name__ == "__main__":

@@ -47,3 +47,24 @@ def test_gpu_sa_every_evaluation_checked_against_oracle():
 
     got = SA.sa_general(G.random_regular(n, d, 1), n, d, 7, 2000, evaluator=both)
     assert checked[0] == got.evaluations >= 2001
+
+
+@pytest.mark.parametrize("case_id", ["gsa:c4", "gsa:w10h10d4r3g2", "gsa:w10h10d4r3g4",
+                                     "gsa:w8h8d6r3g4:hill"])
+def test_gpu_grid_sa_matches_reference_trajectory(case_id):
+    """The grid driver on the device evaluator (rotate_hash relabelling in the
+    prep kernel) visits the reference grid sa()'s graphs."""
+    from cases import GOLDEN_GRID_SA, grid_sa_cases
+
+    with open(GOLDEN_GRID_SA) as f:
+        row = {r["id"]: r for r in json.load(f)}[case_id]
+    case = {c["id"]: c for c in grid_sa_cases()}[case_id]
+    w, h, d, r, g = case["width"], case["height"], case["degree"], case["length"], case["groups"]
+    e0 = G.random_grid_symmetric(w, h, d, r, g, case["graph_seed"])
+    got = SA.sa_grid(e0, w, h, d, r, case["seed"], case["ncalcs"], groups=g,
+                     cooling_cycle=case.get("cooling_cycle", 1),
+                     hill_climbing=case.get("hill", False), fixed_temp=case.get("fixed_temp", 0.0))
+    assert got.steps == row["steps"]
+    assert got.best_diameter == row["best_diameter"]
+    assert got.best_aspl == float.fromhex(row["best_aspl_hex"])
+    assert hashlib.sha256(got.edges.tobytes()).hexdigest() == row["edges_sha256"]
