@@ -39,6 +39,7 @@ WORKLOADS = {
     "c1": "apsp n=256 d=8 g=1 (random regular, seed 1)",
     "c4": "apsp grid 100x100 d=4 r=2 (seed 1)",
     "c3": "batch of 512 graphs n~U[1024,16384] d in {8,12,16,24,32} (one 2-opt each)",
+    "c4b": "batch of 256 length-respecting 2-opt proposals of the grid 100x100 d=4 r=2 (seed 1)",
 }
 
 
@@ -68,6 +69,14 @@ def build_workload(name):
             e = two_opt(G.random_regular(n, d, s), n, np.random.default_rng(s), 1)
             out.append(dict(adj=G.edges_to_adj(n, d, e), based_nodes=n, groups=1, symmetric=True))
         return out
+    if name == "c4b":
+        sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+        from cases import grid_two_opt_variants
+
+        e0 = G.random_grid(100, 100, 4, 2, 1)
+        variants = grid_two_opt_variants(e0, 100, 2, np.random.default_rng(4), 256)
+        return [dict(adj=G.edges_to_adj(10000, 4, e), based_nodes=10000, groups=1, symmetric=True)
+                for e in variants]
     raise SystemExit(f"unknown workload {name}")
 
 

@@ -201,10 +201,61 @@ struct SaGraph {
       p.r = (int)rng.below(2);
     } while (!load(p));
   }
-  // True when neither added edge exists yet (the move keeps the graph simple).
+  // Grid graphs: line t1 is drawn among the lines at a vertex within `length`
+  // steps of line t0's first endpoint, so that most draws respect the limit.
+  std::vector<int32_t> lineat;  // lineat[v*d + j]: the line holding edge (v, adj[v][j])
+  std::vector<int> offsets;     // packed (dx, dy) of the length ball
+  void grid_setup() {
+    lineat.assign((size_t)n * d, -1);
+    for (int i = 0; i < lines; ++i)
+      for (int side = 0; side < 2; ++side) {
+        const int u = edges[2 * i + side], v = edges[2 * i + 1 - side];
+        for (int j = 0; j < d; ++j)
+          if (adj[(size_t)u * d + j] == v && lineat[(size_t)u * d + j] < 0) {
+            lineat[(size_t)u * d + j] = i;
+            break;
+          }
+      }
+    offsets.clear();
+    for (int dx = -grid_length; dx <= grid_length; ++dx)
+      for (int dy = -grid_length; dy <= grid_length; ++dy)
+        if ((dx || dy) && std::abs(dx) + std::abs(dy) <= grid_length)
+          offsets.push_back((dx + 1024) * 4096 + (dy + 1024));
+  }
+  bool draw_local(Rng &rng, Prop &p) const {
+    p.t0 = (int)rng.below((uint32_t)lines);
+    const int a = edges[2 * p.t0];
+    const int o = offsets[rng.below((uint32_t)offsets.size())];
+    const int cx = a / grid_height + (o / 4096 - 1024), cy = a % grid_height + (o % 4096 - 1024);
+    if (cx < 0 || cy < 0 || cy >= grid_height || cx >= n / grid_height) return false;
+    const int c = cx * grid_height + cy;
+    p.t1 = lineat[(size_t)c * d + rng.below((uint32_t)d)];
+    p.r = (int)rng.below(2);
+    return load(p);
+  }
+  // A valid move (simple graph kept, grid lengths respected); false after
+  // `tries` failed draws.
+  bool draw_valid(Rng &rng, Prop &p, int tries) const {
+    for (int k = 0; k < tries; ++k) {
+      if (grid_height > 0 ? draw_local(rng, p) : (draw(rng, p), true))
+        if (fresh(p)) return true;
+    }
+    return false;
+  }
+  // Grid graphs (height > 0): vertex v at column v / height, row v % height,
+  // every edge at most `length` Manhattan steps long (grid/sa.c:196-201).
+  int grid_height = 0, grid_length = 0;
+  bool short_enough(int u, int v) const {
+    if (grid_height <= 0) return true;
+    return std::abs(u / grid_height - v / grid_height) + std::abs(u % grid_height - v % grid_height) <=
+           grid_length;
+  }
+  // True when neither added edge exists yet (the move keeps the graph simple)
+  // and, on a grid, both respect the length limit.
   bool fresh(const Prop &p) const {
     const int q1 = p.r ? p.c : p.d, p2 = p.r ? p.b : p.c, q2 = p.r ? p.d : p.b;
-    return !set.has(ekey(p.a, q1)) && !set.has(ekey(p2, q2));
+    return short_enough(p.a, q1) && short_enough(p2, q2) && !set.has(ekey(p.a, q1)) &&
+           !set.has(ekey(p2, q2));
   }
   int find(int y, int v) const {
     for (int x = 0; x < d; ++x)
@@ -227,6 +278,12 @@ struct SaGraph {
   }
   void apply(const Prop &p, const Slots &s) {
     for (int k = 0; k < 4; ++k) adj[s.at[k]] = s.val[k];
+    if (!lineat.empty()) {  // a's and d's (r=0) or a's and c's (r=1) new edges are line t0
+      lineat[s.at[0]] = p.t0;
+      lineat[s.at[1]] = p.t1;
+      lineat[s.at[2]] = p.r ? p.t0 : p.t1;
+      lineat[s.at[3]] = p.r ? p.t1 : p.t0;
+    }
     set.remove(ekey(p.a, p.b));
     set.remove(ekey(p.c, p.d));
     const int e0a = p.a, e0b = p.r ? p.c : p.d, e1a = p.r ? p.b : p.c, e1b = p.r ? p.d : p.b;
@@ -407,6 +464,17 @@ extern "C" int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa
   if (rc) return rc;
   SaGraph g(nodes, degree, edges);
   if ((rc = g.init())) return rc;
+  if (P->grid_height > 0) {
+    if (nodes % P->grid_height || P->grid_length < 1)
+      return set_error(GG_ERR_INVALID, "grid height %d / length %d do not fit %d nodes", P->grid_height,
+                       P->grid_length, nodes);
+    g.grid_height = P->grid_height;
+    g.grid_length = P->grid_length;
+    for (int i = 0; i < g.lines; ++i)
+      if (!g.short_enough(edges[2 * i], edges[2 * i + 1]))
+        return set_error(GG_ERR_INVALID, "line %d is longer than %d", i, P->grid_length);
+    g.grid_setup();
+  }
   std::vector<int32_t> best_edges(edges, edges + 2 * (size_t)g.lines);
   Rng rng(P->seed);
   hipStream_t es = gg::engine_stream();
@@ -486,15 +554,13 @@ extern "C" int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa
       Prop &p = props[i];
       if (explore) {
         Slots s;
-        int tries = 0;
-        for (;;) {
-          g.draw(rng, p);
-          if (g.fresh(p) && g.slots(p, s)) break;
-          if (++tries > 100000) return set_error(GG_ERR_INVALID, "no valid 2-opt move found");
-        }
+        if (!g.draw_valid(rng, p, 1000000) || !g.slots(p, s))
+          return set_error(GG_ERR_INVALID, "no valid 2-opt move found");
         cand[i].nm = 1;
         cand[i].prop[0] = i;
         cand[i].sl[0] = s;
+      } else if (g.grid_height > 0) {
+        if (!g.draw_valid(rng, p, 1000000)) return set_error(GG_ERR_INVALID, "no valid 2-opt move found");
       } else {
         g.draw(rng, p);
       }

@@ -118,7 +118,8 @@ class BsaParams(ctypes.Structure):
                 ("min_temp", ctypes.c_double), ("seed", ctypes.c_uint64),
                 ("train_steps", ctypes.c_int32), ("batch_train", ctypes.c_int32),
                 ("lr", ctypes.c_float), ("hist_maxd", ctypes.c_int32),
-                ("feature_refresh", ctypes.c_int32), ("log_capacity", ctypes.c_int32)]
+                ("feature_refresh", ctypes.c_int32), ("log_capacity", ctypes.c_int32),
+                ("grid_height", ctypes.c_int32), ("grid_length", ctypes.c_int32)]
 
 
 class BsaStats(ctypes.Structure):
@@ -150,10 +151,12 @@ def sa_batched(edges, nodes: int, degree: int, *, max_steps: int = 100, time_bud
                max_temp: float = 100.0, min_temp: float = 0.217147, seed: int = 1,
                train_steps: int = 1, batch_train: int = 256, lr: float = 1e-3, hist_maxd: int = 8,
                feature_refresh: int = 100, log_capacity: int = 100000,
-               accepted_capacity: int = 100000) -> BatchedResult:
+               accepted_capacity: int = 100000, grid_height: int = 0,
+               grid_length: int = 0) -> BatchedResult:
     """Batched (and, with use_surrogate, MLP-ranked) SA on the GPU; see
     include/gg_host.h.  exact = 1 without the surrogate is plain single-graph
-    GPU SA with a device-resident adjacency."""
+    GPU SA with a device-resident adjacency.  grid_height > 0 keeps every
+    edge of a (groups = 1) grid graph within grid_length Manhattan steps."""
     lib = L.lib()
     if not getattr(lib, "_bsa_sigs", False):
         lib.gg_sa_batched.argtypes = [ctypes.c_int, ctypes.c_int, L.i32p,
@@ -169,7 +172,8 @@ def sa_batched(edges, nodes: int, degree: int, *, max_steps: int = 100, time_bud
     e = L.c32(edges).reshape(-1, 2).copy()
     p = BsaParams(max_steps, time_budget_s, target_aspl, proposals, exact, combine, warmup,
                   explore_every, int(use_surrogate), int(random_ranking), max_temp, min_temp,
-                  seed, train_steps, batch_train, lr, hist_maxd, feature_refresh, log_capacity)
+                  seed, train_steps, batch_train, lr, hist_maxd, feature_refresh, log_capacity,
+                  grid_height, grid_length)
     st = BsaStats()
     lt = np.zeros(max(log_capacity, 1))
     la = np.zeros(max(log_capacity, 1))

@@ -151,6 +151,8 @@ typedef struct gg_bsa_params {
   int32_t hist_maxd;      /* distance histogram depth for the statistics (<= 15) */
   int32_t feature_refresh;/* recompute per-vertex statistics every k accepts */
   int32_t log_capacity;   /* entries in the log arrays */
+  int32_t grid_height;    /* > 0: a grid graph, vertex v at (v / height, v % height) */
+  int32_t grid_length;    /* grid: max Manhattan length of every edge (grid/sa.c:196-201) */
 } gg_bsa_params;
 
 typedef struct gg_bsa_stats {

@@ -141,3 +141,22 @@ def test_gpu_surrogate_sa_full_size_accepts_are_exact():
             assert (g.diameter, g.sum) == (res.accepted_diam[k], int(res.accepted_sum[k])), k
     o = oracle.matrix_op(G.edges_to_adj(n, d, res.edges))
     assert o.diameter == res.stats["best_diameter"] and o.aspl == res.stats["best_aspl"]
+
+
+@pytest.mark.parametrize("surrogate", [False, True])
+def test_gpu_batched_grid_sa_respects_length_and_is_exact(surrogate):
+    """Grid graphs (30x30, d=4, r=2): every scored move keeps each edge within
+    the length limit; every accepted candidate re-scored by the oracle."""
+    w, h, d, r = 30, 30, 4, 2
+    n = w * h
+    e0 = G.random_grid(w, h, d, r, 3)
+    res = SA.sa_batched(e0, n, d, max_steps=60, proposals=512, exact=32, combine=4, warmup=20,
+                        explore_every=4, use_surrogate=surrogate, seed=6, train_steps=2,
+                        batch_train=128, feature_refresh=5, grid_height=h, grid_length=r)
+    assert res.stats["accepts"] > 0
+    replay_and_check(e0, n, d, res)
+    e = e0.copy()
+    for t0, t1, v in res.accepted:
+        SA.apply_move(e, t0, t1, v)
+        x, y = e // h, e % h
+        assert np.all(np.abs(x[:, 0] - x[:, 1]) + np.abs(y[:, 0] - y[:, 1]) <= r)
