@@ -93,6 +93,9 @@ struct GraphOut {
 
 template <typename W> struct WordBits { static constexpr int value = 8 * sizeof(W); };
 
+// Word 3 of a raw buffer resource on gfx9-family parts (32-bit data format).
+constexpr int kBufferWord3 = 0x00020000;
+
 // Pointer into the global address space.
 template <typename T> using gptr = __attribute__((address_space(1))) T *;
 
@@ -235,9 +238,36 @@ __global__ void gg_prep_patched_kernel(const int32_t *__restrict__ base,
 // 16 bytes of an adjacency row: 8 uint16 neighbour ids.
 typedef uint32_t Chunk __attribute__((ext_vector_type(4)));
 
+// Byte offset into a W array of the id held in the low (H = 0) or high
+// (H = 1) 16 bits of x: one SDWA shift per id instead of an extract and a
+// shift.
+template <typename W, int H> __device__ __forceinline__ uint32_t id_off(uint32_t x) {
+  constexpr int S = sizeof(W) == 1 ? 0 : sizeof(W) == 2 ? 1 : sizeof(W) == 4 ? 2 : 3;
+  uint32_t r;
+  if constexpr (H == 0)
+    asm("v_lshlrev_b32_sdwa %0, %2, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD "
+        "src1_sel:WORD_0"
+        : "=v"(r)
+        : "v"(x), "n"(S));
+  else
+    asm("v_lshlrev_b32_sdwa %0, %2, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD "
+        "src1_sel:WORD_1"
+        : "=v"(r)
+        : "v"(x), "n"(S));
+  return r;
+}
+
+// The reach array R starts at LDS address 0 (the kernel checks this on
+// entry), so an id's byte offset is its LDS address and the gathers need no
+// base add.
+template <typename W> __device__ __forceinline__ W word_at(uint32_t lds_addr) {
+  return *(const __attribute__((address_space(3))) W *)(size_t)lds_addr;
+}
+
 // OR of the reach words of 4 neighbour ids (two packed uint32 pairs).
-template <typename W> __device__ __forceinline__ W gather4(const W *R, uint32_t a, uint32_t b) {
-  return (W)(R[a & 0xffffu] | R[a >> 16] | R[b & 0xffffu] | R[b >> 16]);
+template <typename W> __device__ __forceinline__ W gather4(const W *, uint32_t a, uint32_t b) {
+  return (W)(word_at<W>(id_off<W, 0>(a)) | word_at<W>(id_off<W, 1>(a)) |
+             word_at<W>(id_off<W, 0>(b)) | word_at<W>(id_off<W, 1>(b)));
 }
 
 // ORs a chunk's 8 neighbours into val, in two halves, stopping once full.
@@ -263,6 +293,7 @@ gg_tile_bfs_kernel(const GraphDev *__restrict__ graphs, const UnitDesc *__restri
   const int n = G.n, dp = G.dp, split = G.split;
   const int nch = dp >> 3;  // 16-byte chunks per adjacency row
   const int c = n - split;
+  const bool has_centres = c > 0;
   // The descriptor's pointer is generic; say it is global so the gathers
   // compile to global_load (not flat_load).
   const gptr<const Chunk> rows = (gptr<const Chunk>)G.adjR;
@@ -281,6 +312,15 @@ gg_tile_bfs_kernel(const GraphDev *__restrict__ graphs, const UnitDesc *__restri
   uint32_t *qcount = (uint32_t *)(smem + off);  // [2]
 
   const int32_t bad = G.flags[0];
+  const bool lds_base_ok =
+      (uint32_t)(size_t)(const __attribute__((address_space(3))) unsigned char *)smem == 0u;
+  if (!lds_base_ok) {  // uniform: the gathers address R from LDS address 0
+    if (tid == 0) {
+      UnitOut o = {0, 0, n + 1, 0, 1, 2};
+      outs[uidx] = o;
+    }
+    return;
+  }
   if (bad != 0) {  // uniform: adjacency had out-of-range entries
     if (tid == 0) {
       UnitOut o = {0, 0, n + 1, 0, 1, 1};
@@ -402,58 +442,104 @@ gg_tile_bfs_kernel(const GraphDev *__restrict__ graphs, const UnitDesc *__restri
       }
     } else {
       // Bottom-up: every vertex ORs its neighbours' words, stopping once its
-      // own word is full (tested every 4 ids).  The first two chunks (16 ids)
-      // of the rows of the next two vertices are in flight while the current
-      // vertex gathers.  `t` is laundered so that per-vertex addresses are
-      // recomputed inside this level instead of being hoisted out of the
-      // level loop (which would pin VPT of them in registers).
+      // own word is full (tested every 4 ids on dense levels).  `t` is
+      // laundered so that per-vertex addresses are recomputed inside this
+      // level instead of being hoisted out of the level loop (which would pin
+      // VPT of them in registers).
       int t = tid;
       asm volatile("" : "+v"(t));
-      const Chunk zero = {0u, 0u, 0u, 0u};
-      W o0 = (t < n) ? R[t] : full, o1 = full;
-      Chunk a0 = zero, b0 = zero, a1 = zero, b1 = zero;
-      if (o0 != full) {
-        a0 = rows[(size_t)t * nch];
-        if (nch > 1) b0 = rows[(size_t)t * nch + 1];
-      }
-      if (VPT > 1) {
-        const int v1 = t + BLOCK;
-        o1 = (v1 < n) ? R[v1] : full;
-        if (o1 != full) {
-          a1 = rows[(size_t)v1 * nch];
-          if (nch > 1) b1 = rows[(size_t)v1 * nch + 1];
-        }
-      }
-#pragma unroll
+      const uint32_t row_bytes = (uint32_t)nch * 16u;
+      const gptr<const char> row_base = (gptr<const char>)G.adjR;
+      const __amdgpu_buffer_rsrc_t row_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+          (void *)G.adjR, (short)0, (int)((uint32_t)n * row_bytes), kBufferWord3);
+      // Below a quarter of the source-vertex pairs reached, a vertex rarely
+      // fills up within its first 16 neighbours: rows of two chunks then
+      // gather all 16 without the per-4 early-exit tests (more loads in
+      // flight per wave, straight-line code).
+      const bool sparse2 =
+          nch == 2 && 4ull * (uint64_t)(tot_lo + tot_hi) < (uint64_t)(lo_full + hi_full);
+      W chg_acc = (W)0;
+      uint32_t p_all = 0;
+      // Buffer loads of a row's first two chunks (the second one is always
+      // loaded: for a one-chunk row it is the next row's first chunk, or zeros
+      // past the table, and is not used).  A full vertex (or one past n) gets
+      // an offset outside the buffer, which the hardware answers with zeros
+      // without touching memory, so the loads need no branch and the compiler
+      // waits for exactly the row being used.
+      auto fetch_rows = [&](int v, W o, Chunk &x, Chunk &y) {
+        const uint32_t off = o != full ? (uint32_t)v * row_bytes : 0x80000000u;
+        x = __builtin_amdgcn_raw_buffer_load_b128(row_rsrc, off, 0, 0);
+        y = __builtin_amdgcn_raw_buffer_load_b128(row_rsrc, off + 16u, 0, 0);
+      };
+      // Software pipeline over this thread's vertices: the word of vertex
+      // r + 3 is read from LDS while vertex r gathers, so that the row load
+      // of vertex r + 2 (issued before vertex r gathers; out of range, hence
+      // free of memory traffic, for a full vertex) does not wait behind the
+      // gathers in the LDS queue.  Words rotate through four slots and rows
+      // through three; the loop is fully unrolled, so every slot choice is
+      // resolved at compile time.
+      W o0 = full, o1 = full, o2 = full, o3 = full;
+      Chunk a0, b0, a1, b1, a2, b2;
+      auto word = [&](int v) { return (v < n) ? R[v] : full; };
+      o0 = word(t);
+      if (VPT > 1) o1 = word(BLOCK + t);
+      if (VPT > 2) o2 = word(2 * BLOCK + t);
+      fetch_rows(t, o0, a0, b0);
+      if (VPT > 1) fetch_rows(BLOCK + t, o1, a1, b1);
+#pragma clang loop unroll(full)
       for (int r = 0; r < VPT; ++r) {
         const int v = r * BLOCK + t;
-        const W old = o0;
-        const Chunk ca = a0, cb = b0;
-        o0 = o1;
-        a0 = a1;
-        b0 = b1;
+        if (r + 3 < VPT) {
+          const W o = word(v + 3 * BLOCK);
+          const int q = (r + 3) % 4;
+          if (q == 0) o0 = o;
+          else if (q == 1) o1 = o;
+          else if (q == 2) o2 = o;
+          else o3 = o;
+        }
         if (r + 2 < VPT) {
-          const int v2 = v + 2 * BLOCK;
-          o1 = (v2 < n) ? R[v2] : full;
-          if (o1 != full) {
-            a1 = rows[(size_t)v2 * nch];
-            if (nch > 1) b1 = rows[(size_t)v2 * nch + 1];
+          const int q = (r + 2) % 4;
+          const W o = q == 0 ? o0 : q == 1 ? o1 : q == 2 ? o2 : o3;
+          Chunk x, y;
+          fetch_rows(v + 2 * BLOCK, o, x, y);
+          const int s2 = (r + 2) % 3;
+          if (s2 == 0) {
+            a0 = x;
+            b0 = y;
+          } else if (s2 == 1) {
+            a1 = x;
+            b1 = y;
+          } else {
+            a2 = x;
+            b2 = y;
           }
         }
+        const int q = r % 4, s = r % 3;
+        const W old = q == 0 ? o0 : q == 1 ? o1 : q == 2 ? o2 : o3;
+        const Chunk ca = s == 0 ? a0 : s == 1 ? a1 : a2;
+        const Chunk cb = s == 0 ? b0 : s == 1 ? b1 : b2;
         W val = old;
         if (v < n) {
           if (val != full) {
-            val = gather_chunk<W>(R, ca, val, full);
-            if (nch > 1 && val != full) val = gather_chunk<W>(R, cb, val, full);
-            for (int ch = 2; ch < nch && val != full; ++ch)
-              val = gather_chunk<W>(R, rows[(size_t)v * nch + ch], val, full);
+            if (sparse2) {  // uniform: two-chunk rows on a sparse level, 16 ids at once
+              val = (W)(val | gather4<W>(R, ca.x, ca.y) | gather4<W>(R, ca.z, ca.w) |
+                        gather4<W>(R, cb.x, cb.y) | gather4<W>(R, cb.z, cb.w));
+            } else {
+              val = gather_chunk<W>(R, ca, val, full);
+              if (nch > 1 && val != full) val = gather_chunk<W>(R, cb, val, full);
+              if (nch > 2) {
+                const gptr<const Chunk> rp = (gptr<const Chunk>)(row_base + (uint32_t)v * row_bytes);
+                for (int ch = 2; ch < nch && val != full; ++ch)
+                  val = gather_chunk<W>(R, rp[ch], val, full);
+              }
+            }
           }
           const uint32_t pc = (uint32_t)wpopc<W>(val);
-          if (v < split) p_lo += pc;
-          else p_hi += pc;
-          if (val != old) {
-            p_chg = 1;
-            if constexpr (HIST)
+          p_all += pc;
+          if (has_centres && v >= split) p_hi += pc;  // uniform test first
+          chg_acc = (W)(chg_acc | (val ^ old));
+          if constexpr (HIST) {
+            if (val != old)
               atomicAdd(&G.hist[(size_t)v * (G.hist_maxd + 1) + min(level, G.hist_maxd)],
                         wpopc<W>((W)(val & (W)~old)));
           }
@@ -463,6 +549,8 @@ gg_tile_bfs_kernel(const GraphDev *__restrict__ graphs, const UnitDesc *__restri
         // the prefetches issued above stay in flight across this point.
         __builtin_amdgcn_sched_barrier(0);
       }
+      p_lo = p_all - p_hi;
+      p_chg = chg_acc != (W)0 ? 1u : 0u;
     }
 
     // Block reduction of (lo, hi, changed).
@@ -897,7 +985,8 @@ int assemble(int count, const gg_graph *gs, int hist_maxd, Assembly &A) {
     s.bits = choose_bits(s.n, s.d, s.b, s.g, s.c, count);
     if (s.bits == 0) return set_error(GG_ERR_TOO_LARGE, "n=%d does not fit LDS", s.n);
     block_vpt_for(s.n, &s.block, &s.vpt);
-    if (s.bits == 64 && s.vpt == 64) return set_error(GG_ERR_TOO_LARGE, "no 64-bit instance for n=%d", s.n);
+    if (!kernel_for({s.bits, s.block, s.vpt, 0}))
+      return set_error(GG_ERR_TOO_LARGE, "no %d-bit instance for n=%d", s.bits, s.n);
     s.adj_off = A.adj_elems;
     A.adj_elems += (size_t)s.n * s.d;
     s.adjT_off = A.adjT_elems;
@@ -967,7 +1056,9 @@ int prepare_buffers(Assembly &A, DevSet &D, int hist_maxd, int32_t *hist_dev,
   int rc;
   if ((rc = D.adj_in.ensure(A.adj_elems * 4))) return rc;
   if ((rc = D.relabel_in.ensure(std::max<size_t>(A.rel_elems, 1) * 4))) return rc;
-  if ((rc = D.adjT.ensure(A.adjT_elems * 2))) return rc;
+  // +64 bytes: the pull levels always load a row's first two 16-byte chunks,
+  // so a one-chunk last row reads 16 bytes past the table.
+  if ((rc = D.adjT.ensure(A.adjT_elems * 2 + 64))) return rc;
   if ((rc = D.flags.ensure((size_t)count * 2 * 4))) return rc;
   if ((rc = D.gdesc.ensure((size_t)count * sizeof(GraphDev)))) return rc;
   if ((rc = D.units.ensure(A.units.size() * sizeof(UnitDesc)))) return rc;
@@ -1061,7 +1152,9 @@ int collect(Assembly &A, DevSet &D, gg_result *out) {
     out[i].max_ecc = r[i].max_ecc;
     out[i].word_bits = A.slots[i].bits;
     out[i].sum = r[i].sum;
-    if (r[i].error)
+    if (r[i].error & 2)
+      rc = set_error(GG_ERR_STATE, "graph %zu: the reach array is not at LDS address 0", i);
+    else if (r[i].error)
       rc = set_error(GG_ERR_INVALID, "graph %zu: adjacency entry outside [0,%d)", i, A.slots[i].n);
   }
   return rc;
