@@ -148,10 +148,33 @@ template <typename W> __device__ __forceinline__ W stage_get(const uint32_t *st,
   else return (W)(st[r >> 2] >> ((r & 3) * 8));
 }
 
+// Cross-lane sums with DPP (VALU lane moves) instead of ds_bpermute, which
+// would queue behind the gathers in the LDS pipe.  DPP controls (GFX9):
+// quad_perm [1,0,3,2] = 0xB1, [2,3,0,1] = 0x4E; row_shr:k = 0x110 + k;
+// row_ror:k = 0x120 + k; row_bcast:15 = 0x142; row_bcast:31 = 0x143.
+#define GG_DPP(src, ctrl, rm, bm) __builtin_amdgcn_update_dpp(0u, (src), (ctrl), (rm), (bm), true)
+
+// Sum over the 64 lanes, returned in every lane (uniform).
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+  v += GG_DPP(v, 0xB1, 0xf, 0xf);
+  v += GG_DPP(v, 0x4E, 0xf, 0xf);
+  v += GG_DPP(v, 0x124, 0xf, 0xf);
+  v += GG_DPP(v, 0x128, 0xf, 0xf);  // every lane: its 16-lane row's sum
+  v += GG_DPP(v, 0x142, 0xa, 0xf);  // rows 1, 3 += rows 0, 2
+  v += GG_DPP(v, 0x143, 0xc, 0xf);  // rows 2, 3 += rows 0-1
+  return __builtin_amdgcn_readlane(v, 63);
+}
+
+// Inclusive prefix sum over the lanes (lane i: sum of lanes 0..i).
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+  uint32_t t = x + GG_DPP(x, 0x111, 0xf, 0xf);
+  t += GG_DPP(x, 0x112, 0xf, 0xf);
+  t += GG_DPP(x, 0x113, 0xf, 0xf);  // lanes i-3..i of the row
+  t += GG_DPP(t, 0x114, 0xf, 0xe);  // lanes 4-15 of a row: += 4 back
+  t += GG_DPP(t, 0x118, 0xf, 0xc);  // lanes 8-15: += 8 back (row prefix done)
+  t += GG_DPP(t, 0x142, 0xa, 0xf);  // rows 1, 3 += last lane of rows 0, 2
+  t += GG_DPP(t, 0x143, 0xc, 0xf);  // rows 2, 3 += lane 31
+  return t;
 }
 
 // ---------------------------------------------------------------------------
@@ -417,15 +440,11 @@ gg_tile_bfs_kernel(const GraphDev *__restrict__ graphs, const UnitDesc *__restri
           if (build) {
             // One queue reservation per wave: exclusive scan of the lanes'
             // counts, the last active lane reserves the total.
-            uint32_t incl = mine;
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-              const uint32_t y = __shfl_up(incl, o, 64);
-              if (lane >= o) incl += y;
-            }
+            const uint32_t incl = wave_incl_scan(mine);
+            const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
             uint32_t base = 0;
-            if (lane == 63 && incl) base = atomicAdd(qc_n, incl);
-            base = __shfl(base, 63, 64);
+            if (lane == 63 && total) base = atomicAdd(qc_n, total);
+            base = __builtin_amdgcn_readlane(base, 63);
             uint32_t slot = base + incl - mine;
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
