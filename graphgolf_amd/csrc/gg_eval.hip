@@ -572,10 +572,11 @@ gg_tile_bfs_kernel(const GraphDev *__restrict__ graphs, const UnitDesc *__restri
       p_chg = chg_acc != (W)0 ? 1u : 0u;
     }
 
-    // Block reduction of (lo, hi, changed).
+    // Block reduction of (lo, hi, changed): per wave with DPP, across waves
+    // through LDS (one 16-byte record per wave, double-buffered by level).
     p_lo = wave_sum(p_lo);
     p_hi = wave_sum(p_hi);
-    p_chg = wave_sum(p_chg);
+    p_chg = __builtin_amdgcn_ballot_w64(p_chg != 0u) != 0ull ? 1u : 0u;
     uint32_t *rb = red + (level & 1) * 16 * 4;
     if (lane == 0) {
       rb[wave * 4 + 0] = p_lo;
@@ -592,13 +593,15 @@ gg_tile_bfs_kernel(const GraphDev *__restrict__ graphs, const UnitDesc *__restri
         if (v < n) R[v] = stage_get<W>(st, r);
       }
     }
-    uint32_t L_lo = 0, L_hi = 0, L_chg = 0;
-#pragma unroll
-    for (int w = 0; w < NWAVES; ++w) {
-      L_lo += rb[w * 4 + 0];
-      L_hi += rb[w * 4 + 1];
-      L_chg += rb[w * 4 + 2];
+    // Lane k < NWAVES reads wave k's record; DPP sums finish the reduction.
+    uint32_t r_lo = 0, r_hi = 0, r_chg = 0;
+    if (lane < NWAVES) {
+      const Chunk rec = *(const Chunk *)(rb + lane * 4);
+      r_lo = rec.x;
+      r_hi = rec.y;
+      r_chg = rec.z;
     }
+    const uint32_t L_lo = wave_sum(r_lo), L_hi = wave_sum(r_hi), L_chg = wave_sum(r_chg);
     const uint32_t qn_next = qcount[cur ^ 1];
     __syncthreads();  // B2: R write-back visible, counters read
 
