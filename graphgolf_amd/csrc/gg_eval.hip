@@ -477,6 +477,10 @@ gg_tile_bfs_kernel(const GraphDev *__restrict__ graphs, const UnitDesc *__restri
       // flight per wave, straight-line code).
       const bool sparse2 =
           nch == 2 && 4ull * (uint64_t)(tot_lo + tot_hi) < (uint64_t)(lo_full + hi_full);
+      // Fewer than 1/32 of the source-vertex pairs still unreached.
+      const bool very_dense =
+          32ull * ((uint64_t)(lo_full + hi_full) - (uint64_t)(tot_lo + tot_hi)) <
+          (uint64_t)(lo_full + hi_full);
       W chg_acc = (W)0;
       uint32_t p_all = 0;
       // Buffer loads of a row's first two chunks (the second one is always
@@ -497,76 +501,105 @@ gg_tile_bfs_kernel(const GraphDev *__restrict__ graphs, const UnitDesc *__restri
       // gathers in the LDS queue.  Words rotate through four slots and rows
       // through three; the loop is fully unrolled, so every slot choice is
       // resolved at compile time.
-      W o0 = full, o1 = full, o2 = full, o3 = full;
-      Chunk a0, b0, a1, b1, a2, b2;
-      auto word = [&](int v) { return (v < n) ? R[v] : full; };
-      o0 = word(t);
-      if (VPT > 1) o1 = word(BLOCK + t);
-      if (VPT > 2) o2 = word(2 * BLOCK + t);
-      fetch_rows(t, o0, a0, b0);
-      if (VPT > 1) fetch_rows(BLOCK + t, o1, a1, b1);
+      if (very_dense) {
+        // Almost every vertex is already full: no prefetching; the few that
+        // are not load their row when they get to it.
+        const gptr<const Chunk> rows_g = (gptr<const Chunk>)G.adjR;
 #pragma clang loop unroll(full)
-      for (int r = 0; r < VPT; ++r) {
-        const int v = r * BLOCK + t;
-        if (r + 3 < VPT) {
-          const W o = word(v + 3 * BLOCK);
-          const int q = (r + 3) % 4;
-          if (q == 0) o0 = o;
-          else if (q == 1) o1 = o;
-          else if (q == 2) o2 = o;
-          else o3 = o;
-        }
-        if (r + 2 < VPT) {
-          const int q = (r + 2) % 4;
-          const W o = q == 0 ? o0 : q == 1 ? o1 : q == 2 ? o2 : o3;
-          Chunk x, y;
-          fetch_rows(v + 2 * BLOCK, o, x, y);
-          const int s2 = (r + 2) % 3;
-          if (s2 == 0) {
-            a0 = x;
-            b0 = y;
-          } else if (s2 == 1) {
-            a1 = x;
-            b1 = y;
-          } else {
-            a2 = x;
-            b2 = y;
-          }
-        }
-        const int q = r % 4, s = r % 3;
-        const W old = q == 0 ? o0 : q == 1 ? o1 : q == 2 ? o2 : o3;
-        const Chunk ca = s == 0 ? a0 : s == 1 ? a1 : a2;
-        const Chunk cb = s == 0 ? b0 : s == 1 ? b1 : b2;
-        W val = old;
-        if (v < n) {
+        for (int r = 0; r < VPT; ++r) {
+          const int v = r * BLOCK + t;
+          const W old = (v < n) ? R[v] : full;
+          W val = old;
           if (val != full) {
-            if (sparse2) {  // uniform: two-chunk rows on a sparse level, 16 ids at once
-              val = (W)(val | gather4<W>(R, ca.x, ca.y) | gather4<W>(R, ca.z, ca.w) |
-                        gather4<W>(R, cb.x, cb.y) | gather4<W>(R, cb.z, cb.w));
-            } else {
-              val = gather_chunk<W>(R, ca, val, full);
-              if (nch > 1 && val != full) val = gather_chunk<W>(R, cb, val, full);
-              if (nch > 2) {
-                const gptr<const Chunk> rp = (gptr<const Chunk>)(row_base + (uint32_t)v * row_bytes);
-                for (int ch = 2; ch < nch && val != full; ++ch)
-                  val = gather_chunk<W>(R, rp[ch], val, full);
-              }
+            for (int ch = 0; ch < nch && val != full; ++ch)
+              val = gather_chunk<W>(R, rows_g[(uint32_t)v * (uint32_t)nch + (uint32_t)ch], val, full);
+          }
+          if (v < n) {
+            const uint32_t pc = (uint32_t)wpopc<W>(val);
+            p_all += pc;
+            if (has_centres && v >= split) p_hi += pc;  // uniform test first
+            chg_acc = (W)(chg_acc | (val ^ old));
+            if constexpr (HIST) {
+              if (val != old)
+                atomicAdd(&G.hist[(size_t)v * (G.hist_maxd + 1) + min(level, G.hist_maxd)],
+                          wpopc<W>((W)(val & (W)~old)));
             }
           }
-          const uint32_t pc = (uint32_t)wpopc<W>(val);
-          p_all += pc;
-          if (has_centres && v >= split) p_hi += pc;  // uniform test first
-          chg_acc = (W)(chg_acc | (val ^ old));
-          if constexpr (HIST) {
-            if (val != old)
-              atomicAdd(&G.hist[(size_t)v * (G.hist_maxd + 1) + min(level, G.hist_maxd)],
-                        wpopc<W>((W)(val & (W)~old)));
-          }
+          stage_put<W>(st, r, val);
+          __builtin_amdgcn_sched_barrier(0);
         }
-        stage_put<W>(st, r, val);
-        // Keep the scheduler from interleaving vertices (register pressure);
-        // the prefetches issued above stay in flight across this point.
-        __builtin_amdgcn_sched_barrier(0);
+      } else {
+        W o0 = full, o1 = full, o2 = full, o3 = full;
+        Chunk a0, b0, a1, b1, a2, b2;
+        auto word = [&](int v) { return (v < n) ? R[v] : full; };
+        o0 = word(t);
+        if (VPT > 1) o1 = word(BLOCK + t);
+        if (VPT > 2) o2 = word(2 * BLOCK + t);
+        fetch_rows(t, o0, a0, b0);
+        if (VPT > 1) fetch_rows(BLOCK + t, o1, a1, b1);
+  #pragma clang loop unroll(full)
+        for (int r = 0; r < VPT; ++r) {
+          const int v = r * BLOCK + t;
+          if (r + 3 < VPT) {
+            const W o = word(v + 3 * BLOCK);
+            const int q = (r + 3) % 4;
+            if (q == 0) o0 = o;
+            else if (q == 1) o1 = o;
+            else if (q == 2) o2 = o;
+            else o3 = o;
+          }
+          if (r + 2 < VPT) {
+            const int q = (r + 2) % 4;
+            const W o = q == 0 ? o0 : q == 1 ? o1 : q == 2 ? o2 : o3;
+            Chunk x, y;
+            fetch_rows(v + 2 * BLOCK, o, x, y);
+            const int s2 = (r + 2) % 3;
+            if (s2 == 0) {
+              a0 = x;
+              b0 = y;
+            } else if (s2 == 1) {
+              a1 = x;
+              b1 = y;
+            } else {
+              a2 = x;
+              b2 = y;
+            }
+          }
+          const int q = r % 4, s = r % 3;
+          const W old = q == 0 ? o0 : q == 1 ? o1 : q == 2 ? o2 : o3;
+          const Chunk ca = s == 0 ? a0 : s == 1 ? a1 : a2;
+          const Chunk cb = s == 0 ? b0 : s == 1 ? b1 : b2;
+          W val = old;
+          if (v < n) {
+            if (val != full) {
+              if (sparse2) {  // uniform: two-chunk rows on a sparse level, 16 ids at once
+                val = (W)(val | gather4<W>(R, ca.x, ca.y) | gather4<W>(R, ca.z, ca.w) |
+                          gather4<W>(R, cb.x, cb.y) | gather4<W>(R, cb.z, cb.w));
+              } else {
+                val = gather_chunk<W>(R, ca, val, full);
+                if (nch > 1 && val != full) val = gather_chunk<W>(R, cb, val, full);
+                if (nch > 2) {
+                  const gptr<const Chunk> rp = (gptr<const Chunk>)(row_base + (uint32_t)v * row_bytes);
+                  for (int ch = 2; ch < nch && val != full; ++ch)
+                    val = gather_chunk<W>(R, rp[ch], val, full);
+                }
+              }
+            }
+            const uint32_t pc = (uint32_t)wpopc<W>(val);
+            p_all += pc;
+            if (has_centres && v >= split) p_hi += pc;  // uniform test first
+            chg_acc = (W)(chg_acc | (val ^ old));
+            if constexpr (HIST) {
+              if (val != old)
+                atomicAdd(&G.hist[(size_t)v * (G.hist_maxd + 1) + min(level, G.hist_maxd)],
+                          wpopc<W>((W)(val & (W)~old)));
+            }
+          }
+          stage_put<W>(st, r, val);
+          // Keep the scheduler from interleaving vertices (register pressure);
+          // the prefetches issued above stay in flight across this point.
+          __builtin_amdgcn_sched_barrier(0);
+        }
       }
       p_lo = p_all - p_hi;
       p_chg = chg_acc != (W)0 ? 1u : 0u;
