@@ -185,11 +185,17 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
 // are replaced by the row's own id (memory-safe) and counted in flags[0];
 // flags[1] counts asymmetric entries when check_sym is set.
 
-__global__ void gg_prep_kernel(const int32_t *__restrict__ adj, const int32_t *__restrict__ relabel,
-                               int n, int d, int dp, uint16_t *__restrict__ adjR,
-                               int32_t *__restrict__ flags, int check_sym) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+struct PrepDesc {  // one graph of a prep launch
+  const int32_t *adj, *relabel;
+  uint16_t *adjR;
+  int32_t *flags;
+  int n, d, dp, check_sym;
+};
+
+__device__ __forceinline__ void prep_row(const int32_t *__restrict__ adj,
+                                         const int32_t *__restrict__ relabel, int n, int d, int dp,
+                                         uint16_t *__restrict__ adjR, int32_t *__restrict__ flags,
+                                         int check_sym, int i) {
   const int ri = relabel ? relabel[i] : i;
   const int32_t *row = adj + (size_t)i * d;
   uint16_t *out = adjR + (size_t)ri * dp;
@@ -217,6 +223,13 @@ __global__ void gg_prep_kernel(const int32_t *__restrict__ adj, const int32_t *_
   for (int j = d; j < dp; ++j) out[j] = (uint16_t)ri;
   if (bad) atomicAdd(&flags[0], bad);
   if (asym) atomicAdd(&flags[1], asym);
+}
+
+// All graphs of a launch set in one launch: blockIdx.y selects the graph.
+__global__ void gg_prep_kernel(const PrepDesc *__restrict__ descs) {
+  const PrepDesc P = descs[blockIdx.y];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < P.n) prep_row(P.adj, P.relabel, P.n, P.d, P.dp, P.adjR, P.flags, P.check_sym, i);
 }
 
 // Patched variant for batches of proposals that differ from one base graph
@@ -1096,10 +1109,11 @@ int assemble(int count, const gg_graph *gs, int hist_maxd, Assembly &A) {
 
 // Device buffers of one launch set (either the engine's single scratch or a plan's own).
 struct DevSet {
-  DevBuf adj_in, relabel_in, adjT, flags, gdesc, units, outs, meta, gout, hist;
+  DevBuf adj_in, relabel_in, adjT, flags, gdesc, units, outs, meta, gout, hist, prep;
   PinBuf res_h;
   void release() {
-    for (DevBuf *b : {&adj_in, &relabel_in, &adjT, &flags, &gdesc, &units, &outs, &meta, &gout, &hist})
+    for (DevBuf *b : {&adj_in, &relabel_in, &adjT, &flags, &gdesc, &units, &outs, &meta, &gout, &hist,
+                      &prep})
       b->release();
     res_h.release();
   }
@@ -1145,6 +1159,25 @@ int prepare_buffers(Assembly &A, DevSet &D, int hist_maxd, int32_t *hist_dev,
   return GG_OK;
 }
 
+// Per-graph prep descriptors for gg_prep_kernel (uploaded before enqueue, so
+// that a captured sequence holds no host-to-device copy of them).
+int upload_prep(Assembly &A, DevSet &D, const int32_t *const *adj_src, const int32_t *const *rel_src,
+                const int *check_sym) {
+  const int count = (int)A.slots.size();
+  std::vector<PrepDesc> h(count);
+  for (int i = 0; i < count; ++i) {
+    const Slot &s = A.slots[i];
+    h[i] = {adj_src[i], rel_src[i], (uint16_t *)D.adjT.p + s.adjT_off, (int32_t *)D.flags.p + 2 * i,
+            s.n, s.d, s.dp, check_sym[i]};
+  }
+  int rc = D.prep.ensure((size_t)count * sizeof(PrepDesc));
+  if (rc) return rc;
+  GG_HIP(hipMemcpyAsync(D.prep.p, h.data(), (size_t)count * sizeof(PrepDesc), hipMemcpyHostToDevice,
+                        E.stream));
+  GG_HIP(hipStreamSynchronize(E.stream));  // h is a local
+  return GG_OK;
+}
+
 // Enqueue prep + tile BFS + combine for an assembled set whose adjacency
 // already sits in `adj_src[i]` (device pointers) -- graph-capturable (no
 // memset nodes, no host sync).
@@ -1171,13 +1204,15 @@ int enqueue(Assembly &A, DevSet &D, const int32_t *const *adj_src, const int32_t
                        patched->base, rel_src[0], patched->patches, patched->max_patch, s.n, s.d,
                        s.dp, (uint16_t *)D.adjT.p, stride, (int32_t *)D.flags.p);
   } else {
-    for (int i = 0; i < count; ++i) {
-      const Slot &s = A.slots[i];
-      hipLaunchKernelGGL(gg_prep_kernel, dim3((s.n + 255) / 256), dim3(256), 0, E.stream,
-                         adj_src[i], rel_src[i], s.n, s.d, s.dp,
-                         (uint16_t *)D.adjT.p + s.adjT_off, (int32_t *)D.flags.p + 2 * i,
-                         check_sym[i]);
-    }
+    // The per-graph descriptors were uploaded by upload_prep (outside any
+    // capture); one launch covers every graph.
+    (void)adj_src;
+    (void)rel_src;
+    (void)check_sym;
+    int max_n = 1;
+    for (const Slot &s : A.slots) max_n = std::max(max_n, s.n);
+    hipLaunchKernelGGL(gg_prep_kernel, dim3((max_n + 255) / 256, count), dim3(256), 0, E.stream,
+                       (const PrepDesc *)D.prep.p);
   }
   if (ev_begin) GG_HIP(hipEventRecord(ev_begin, E.stream));
   for (const LaunchGroup &lg : A.groups) {
@@ -1282,6 +1317,7 @@ int eval_host(int count, const gg_graph *gs, gg_result *out, bool device_ptrs, i
       }
     }
   }
+  if ((rc = upload_prep(A, D, src.data(), rel.data(), sym.data()))) return rc;
   if ((rc = enqueue(A, D, src.data(), rel.data(), sym.data()))) return rc;
   if (hist_maxd > 0)
     GG_HIP(hipMemcpyAsync(hist_host, hist_dev, hist_bytes, hipMemcpyDeviceToHost, E.stream));
@@ -1517,6 +1553,8 @@ static gg_plan *plan_create(int count, const gg_graph *shapes, int max_patch) {
     return plan_fail(p);
   }
   // Capture prep + BFS + combine + result copy once; replays reuse it.
+  if (max_patch == 0 && upload_prep(p->A, p->D, p->src.data(), p->rel.data(), p->sym.data()))
+    return plan_fail(p);
   hipStream_t s = engine().stream;
   if (hipStreamSynchronize(s) != hipSuccess ||
       hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
