@@ -504,7 +504,7 @@ extern "C" int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa
   vs.maxd = std::min(std::max(P->hist_maxd, 2), 15);
   const int cap = 16384;
   const int bt = std::max(P->batch_train, 32);
-  DevArr<int32_t> props_d, idx_d;
+  DevArr<int32_t> props_d, idx_d, tidx_d;
   DevArr<float> vstat_d, feats_d, replay_d, batch_d;
   std::vector<double> by(cap);  // raw labels of the replay rows
   int nbuf = 0, head = 0;
@@ -517,7 +517,7 @@ extern "C" int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa
     if ((rc = props_d.alloc((size_t)npmax * 5)) || (rc = feats_d.alloc((size_t)npmax * kFeat)) ||
         (rc = replay_d.alloc((size_t)cap * kFeat)) || (rc = batch_d.alloc((size_t)bt * kFeat)) ||
         (rc = vstat_d.alloc((size_t)nodes * kVStat)) ||
-        (rc = idx_d.alloc(2 * (size_t)std::max(npmax, bt))))
+        (rc = idx_d.alloc(2 * (size_t)std::max(npmax, bt))) || (rc = tidx_d.alloc((size_t)bt)))
       return rc;
     const double tf = now_s();
     rc = vs.refresh(g);
@@ -535,7 +535,7 @@ extern "C" int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa
   std::vector<Cand> cand(E);
   std::vector<int> cand_verts;
   std::vector<int32_t> src_rows, dst_rows;
-  std::vector<int32_t> idx_h(2 * (size_t)std::max(npmax, bt));
+  std::vector<int32_t> idx_h(2 * (size_t)std::max(npmax, bt)), tidx_h(bt);
   std::vector<float> yb(bt);
   int m_cur = P->combine;
   int since_refresh = 0;
@@ -635,7 +635,31 @@ extern "C" int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa
         }
     const double te = now_s();
     rc = gg_plan_set_patches(plan.p, patches.data());
-    if (!rc) rc = gg_plan_run(plan.p, res.data());
+    if (!rc) rc = gg_plan_launch(plan.p);
+    if (rc) return rc;
+    // While the candidates are scored, train on the rows stored so far (the
+    // network's own stream; this step's rows join the buffer afterwards).
+    if (use_sur && nbuf >= 32 && P->train_steps > 0) {
+      const double tt = now_s();
+      const int b = std::min(bt, nbuf);
+      hipStream_t ms = gg::mlp_stream(mlp.p);
+      for (int s = 0; s < P->train_steps; ++s) {
+        for (int i = 0; i < b; ++i) {
+          const int j = (int)rng.below((uint32_t)nbuf);
+          tidx_h[i] = j;
+          yb[i] = (float)(by[j] / scale);
+        }
+        BSA_HIP(hipMemcpyAsync(tidx_d.p, tidx_h.data(), (size_t)b * 4, hipMemcpyHostToDevice, ms));
+        hipLaunchKernelGGL(gg_bsa_copy_rows, dim3((b * kFeat + 255) / 256), dim3(256), 0, ms,
+                           (const float *)replay_d.p, (const int32_t *)tidx_d.p, batch_d.p,
+                           (const int32_t *)nullptr, b);
+        BSA_HIP(hipGetLastError());
+        float loss = 0;
+        if ((rc = gg::mlp_train_device(mlp.p, batch_d.p, yb.data(), b, P->lr, &loss))) return rc;
+      }
+      t_train += now_s() - tt;
+    }
+    rc = gg_plan_wait(plan.p, res.data());
     t_exact += now_s() - te;
     if (rc) return rc;
     evals += ncand;
@@ -683,25 +707,8 @@ extern "C" int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa
                            (const int32_t *)idx_d.p + rows, rows);
         BSA_HIP(hipGetLastError());
       }
-      const double tt = now_s();
-      const int b = std::min(bt, nbuf);
-      for (int s = 0; s < P->train_steps && nbuf >= 32; ++s) {
-        for (int i = 0; i < b; ++i) {
-          const int j = (int)rng.below((uint32_t)nbuf);
-          idx_h[i] = j;
-          yb[i] = (float)(by[j] / scale);
-        }
-        BSA_HIP(hipMemcpyAsync(idx_d.p, idx_h.data(), (size_t)b * 4, hipMemcpyHostToDevice, es));
-        hipLaunchKernelGGL(gg_bsa_copy_rows, dim3((b * kFeat + 255) / 256), dim3(256), 0, es,
-                           (const float *)replay_d.p, (const int32_t *)idx_d.p, batch_d.p,
-                           (const int32_t *)nullptr, b);
-        BSA_HIP(hipGetLastError());
-        BSA_HIP(hipStreamSynchronize(es));
-        float loss = 0;
-        if ((rc = gg::mlp_train_device(mlp.p, batch_d.p, yb.data(), b, P->lr, &loss))) return rc;
-      }
-      BSA_HIP(hipStreamSynchronize(es));
-      t_train += now_s() - tt;
+      // (Stored on the evaluator's stream, ahead of the next step's feature
+      // kernel that overwrites feats_d.)
       t_sur += now_s() - ts;
     }
     // --- the reference's accept rule (general/sa.c:371-411) on the best candidate

@@ -12,6 +12,8 @@ hipStream_t engine_stream();
 // Forward pass on m rows already in device memory (row-major, in_dim floats
 // each); copies the m predictions to y_host and waits for them.
 int mlp_forward_device(gg_mlp *M, const float *x_dev, int m, float *y_host);
+// The stream the network's kernels run on.
+hipStream_t mlp_stream(gg_mlp *M);
 // One Adam step on m device-resident rows with host targets t_host.
 int mlp_train_device(gg_mlp *M, const float *x_dev, const float *t_host, int m, float lr,
                      float *loss);

@@ -468,6 +468,8 @@ int gg_mlp_train_step(gg_mlp *M, const float *x, const float *t, int m, float lr
 
 namespace gg {
 
+hipStream_t mlp_stream(gg_mlp *M) { return M ? M->stream : nullptr; }
+
 int mlp_forward_device(gg_mlp *M, const float *x_dev, int m, float *y_host) {
   if (!M || !x_dev || !y_host || m < 1) return set_error(GG_ERR_INVALID, "bad arguments");
   int rc = mlp_forward(M, x_dev, m, true);

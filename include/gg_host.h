@@ -160,13 +160,14 @@ typedef struct gg_bsa_stats {
   int32_t best_diameter, pad;
   double best_aspl;
   double seconds_total;
-  double seconds_exact;     /* candidate scoring: patch upload + plan replay */
-  double seconds_surrogate; /* feature kernel, MLP ranking and training */
+  double seconds_exact;     /* candidate scoring: patch upload + plan replay (the
+                               MLP trains on its own stream meanwhile) */
+  double seconds_surrogate; /* feature kernel, MLP ranking, training-row storage */
   double seconds_features;  /* host: proposal draws, candidate assembly, statistics refresh */
   double seconds_to_target; /* -1 when the target was not reached */
   double surrogate_rank_corr; /* Spearman, prediction vs exact dS of explore singles (held out) */
   int64_t moves_accepted;     /* 2-opt moves in accepted candidates */
-  double seconds_train;       /* MLP training (also counted in seconds_surrogate) */
+  double seconds_train;       /* MLP training, overlapped with candidate scoring */
 } gg_bsa_stats;
 
 /* log_time[k], log_best_aspl[k], log_best_diam[k]: after step k.
