@@ -6,6 +6,7 @@ Outputs, next to the package so they travel with the tree:
     graphgolf_amd/libggeval.so     kernels + engine + host SA/graph code (C ABI: include/gg_eval.h)
     graphgolf_amd/libgg_general.so reference ABI shim: general evaluation()  (include/gg_general.h)
     graphgolf_amd/libgg_grid.so    reference ABI shim: grid evaluation()     (include/gg_grid.h)
+    graphgolf_amd/gg_lds_probe     LDS random-gather throughput probe (executable)
 """
 from __future__ import annotations
 
@@ -73,6 +74,10 @@ def build(verbose: bool = False) -> None:
         _run(["g++", *HOST_FLAGS, "-shared", "-o", out + ".tmp", os.path.join(HERE, shim),
               "-L", PKG, "-lggeval", "-Wl,-rpath,$ORIGIN"])
         os.replace(out + ".tmp", out)
+    # The LDS gather-throughput probe (the tile BFS's roofline on this GPU).
+    probe = os.path.join(PKG, "gg_lds_probe")
+    _run([HIPCC, *HIP_FLAGS, os.path.join(HERE, "gg_lds_probe.hip"), "-o", probe + ".tmp"])
+    os.replace(probe + ".tmp", probe)
 
 
 if __name__ == "__main__":

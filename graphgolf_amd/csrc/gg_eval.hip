@@ -192,79 +192,98 @@ struct PrepDesc {  // one graph of a prep launch
   int n, d, dp, check_sym;
 };
 
-__device__ __forceinline__ void prep_row(const int32_t *__restrict__ adj,
-                                         const int32_t *__restrict__ relabel, int n, int d, int dp,
-                                         uint16_t *__restrict__ adjR, int32_t *__restrict__ flags,
-                                         int check_sym, int i) {
-  const int ri = relabel ? relabel[i] : i;
+// One adjacency entry (row i, column j < dp) of a graph: the relabelled id
+// to store at out-row relabel(i), a validity count and a symmetry-mismatch
+// count (0 or 1 each).
+__device__ __forceinline__ uint32_t prep_entry(const int32_t *__restrict__ adj,
+                                               const int32_t *__restrict__ relabel, int n, int d,
+                                               int i, int j, int ri, int check_sym, int *bad,
+                                               int *asym) {
+  if (j >= d) return (uint32_t)ri;  // padding: the row's own id
   const int32_t *row = adj + (size_t)i * d;
-  uint16_t *out = adjR + (size_t)ri * dp;
-  int bad = 0, asym = 0;
-  for (int j = 0; j < d; ++j) {
-    int u = row[j];
-    int ru;
-    if (u < 0 || u >= n) {
-      ++bad;
-      ru = ri;
-    } else {
-      ru = relabel ? relabel[u] : u;
-      if (check_sym) {
-        int c1 = 0, c2 = 0;
-        const int32_t *urow = adj + (size_t)u * d;
-        for (int k = 0; k < d; ++k) {
-          c1 += (row[k] == u);
-          c2 += (urow[k] == i);
-        }
-        asym += (c1 != c2);
-      }
-    }
-    out[j] = (uint16_t)ru;
+  const int u = row[j];
+  if (u < 0 || u >= n) {
+    *bad = 1;
+    return (uint32_t)ri;
   }
-  for (int j = d; j < dp; ++j) out[j] = (uint16_t)ri;
-  if (bad) atomicAdd(&flags[0], bad);
-  if (asym) atomicAdd(&flags[1], asym);
+  if (check_sym) {
+    int c1 = 0, c2 = 0;
+    const int32_t *urow = adj + (size_t)u * d;
+    for (int k = 0; k < d; ++k) {
+      c1 += (row[k] == u);
+      c2 += (urow[k] == i);
+    }
+    *asym = (c1 != c2);
+  }
+  return (uint32_t)(relabel ? relabel[u] : u);
 }
 
-// All graphs of a launch set in one launch: blockIdx.y selects the graph.
-__global__ void gg_prep_kernel(const PrepDesc *__restrict__ descs) {
-  const PrepDesc P = descs[blockIdx.y];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < P.n) prep_row(P.adj, P.relabel, P.n, P.d, P.dp, P.adjR, P.flags, P.check_sym, i);
+// Adds this wave's counts to a graph's flags with one atomic per counter.
+__device__ __forceinline__ void flag_counts(int32_t *flags, int bad, int asym) {
+  const uint64_t mb = __builtin_amdgcn_ballot_w64(bad != 0);
+  const uint64_t ma = __builtin_amdgcn_ballot_w64(asym != 0);
+  const int lane = threadIdx.x & 63;
+  const int first = __builtin_amdgcn_readfirstlane(lane);
+  if (lane == first) {
+    if (mb) atomicAdd(&flags[0], (int)__popcll(mb));
+    if (ma) atomicAdd(&flags[1], (int)__popcll(ma));
+  }
+}
+
+// All graphs of a launch set in one launch, one thread per entry of the
+// padded row-major table (coalesced reads of the int32 rows): blockIdx.y
+// selects the graph.
+__global__ void __launch_bounds__(256) gg_prep_kernel(const PrepDesc *__restrict__ descs) {
+  const PrepDesc P = descs[blockIdx.y];  // descs: this launch's first graph
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = k < (uint32_t)P.n * (uint32_t)P.dp;
+  int bad = 0, asym = 0;
+  if (live) {
+    const int i = (int)(k / (uint32_t)P.dp), j = (int)(k - (uint32_t)i * (uint32_t)P.dp);
+    const int ri = P.relabel ? P.relabel[i] : i;
+    P.adjR[(size_t)ri * P.dp + j] =
+        (uint16_t)prep_entry(P.adj, P.relabel, P.n, P.d, i, j, ri, P.check_sym, &bad, &asym);
+  }
+  if (__builtin_amdgcn_ballot_w64(live) != 0ull) flag_counts(P.flags, bad, asym);
 }
 
 // Patched variant for batches of proposals that differ from one base graph
-// in a few adjacency entries: slot y = blockIdx.y reads the shared base table
-// and overrides the entries listed in patches[y][k] = (v*d + j, new id)
-// (index -1 = unused).  The caller guarantees symmetry (SA moves keep it).
-__global__ void gg_prep_patched_kernel(const int32_t *__restrict__ base,
-                                       const int32_t *__restrict__ relabel,
-                                       const int32_t *__restrict__ patches, int max_patch, int n,
-                                       int d, int dp, uint16_t *__restrict__ adjR0,
-                                       size_t slot_stride, int32_t *__restrict__ flags0) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int slot = blockIdx.y;
-  if (i >= n) return;
-  const int32_t *P = patches + (size_t)slot * max_patch * 2;
-  const int ri = relabel ? relabel[i] : i;
-  const int32_t *row = base + (size_t)i * d;
-  uint16_t *out = adjR0 + slot * slot_stride + (size_t)ri * dp;
-  int bad = 0;
-  for (int j = 0; j < d; ++j) {
-    int u = row[j];
-    const int at = i * d + j;
-    for (int k = 0; k < max_patch; ++k)
-      if (P[2 * k] == at) u = P[2 * k + 1];
-    int ru;
-    if (u < 0 || u >= n) {
-      ++bad;
-      ru = ri;
-    } else {
-      ru = relabel ? relabel[u] : u;
-    }
-    out[j] = (uint16_t)ru;
+// in a few adjacency entries: slot y = blockIdx.y converts the shared base
+// table (entry-parallel, as gg_prep_kernel); gg_patch_kernel then overrides
+// the entries listed in patches[y][k] = (v*d + j, new id) (index -1 =
+// unused).  The caller guarantees symmetry (SA moves keep it).
+__global__ void __launch_bounds__(256)
+gg_prep_patched_kernel(const int32_t *__restrict__ base, const int32_t *__restrict__ relabel, int n,
+                       int d, int dp, uint16_t *__restrict__ adjR0, size_t slot_stride,
+                       int32_t *__restrict__ flags0, int slot0) {
+  const int slot = slot0 + (int)blockIdx.y;
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = k < (uint32_t)n * (uint32_t)dp;
+  int bad = 0, asym = 0;
+  if (live) {
+    const int i = (int)(k / (uint32_t)dp), j = (int)(k - (uint32_t)i * (uint32_t)dp);
+    const int ri = relabel ? relabel[i] : i;
+    adjR0[slot * slot_stride + (size_t)ri * dp + j] =
+        (uint16_t)prep_entry(base, relabel, n, d, i, j, ri, 0, &bad, &asym);
   }
-  for (int j = d; j < dp; ++j) out[j] = (uint16_t)ri;
-  if (bad) atomicAdd(&flags0[2 * slot], bad);
+  if (__builtin_amdgcn_ballot_w64(live) != 0ull) flag_counts(flags0 + 2 * slot, bad, asym);
+}
+
+__global__ void __launch_bounds__(256)
+gg_patch_kernel(const int32_t *__restrict__ patches, int max_patch, int count,
+                const int32_t *__restrict__ relabel, int n, int d, int dp,
+                uint16_t *__restrict__ adjR0, size_t slot_stride, int32_t *__restrict__ flags0) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count * max_patch) return;
+  const int slot = t / max_patch;
+  const int at = patches[2 * t], val = patches[2 * t + 1];
+  if (at < 0 || at >= n * d) return;  // unused entry
+  const int i = at / d, j = at - i * d;
+  const int ri = relabel ? relabel[i] : i;
+  int rv = ri;
+  if (val < 0 || val >= n) atomicAdd(&flags0[2 * slot], 1);
+  else rv = relabel ? relabel[val] : val;
+  adjR0[slot * slot_stride + (size_t)ri * dp + j] = (uint16_t)rv;
 }
 
 // ---------------------------------------------------------------------------
@@ -1200,19 +1219,26 @@ int enqueue(Assembly &A, DevSet &D, const int32_t *const *adj_src, const int32_t
     // All slots share one shape (checked at plan creation) and one base table.
     const Slot &s = A.slots[0];
     const size_t stride = count > 1 ? A.slots[1].adjT_off - A.slots[0].adjT_off : 0;
-    hipLaunchKernelGGL(gg_prep_patched_kernel, dim3((s.n + 255) / 256, count), dim3(256), 0, E.stream,
-                       patched->base, rel_src[0], patched->patches, patched->max_patch, s.n, s.d,
-                       s.dp, (uint16_t *)D.adjT.p, stride, (int32_t *)D.flags.p);
+    const uint32_t entries = (uint32_t)s.n * (uint32_t)s.dp;
+    for (int g0 = 0; g0 < count; g0 += 65535)  // gridDim.y is at most 65535
+      hipLaunchKernelGGL(gg_prep_patched_kernel, dim3((entries + 255) / 256, std::min(65535, count - g0)),
+                         dim3(256), 0, E.stream, patched->base, rel_src[0], s.n, s.d, s.dp,
+                         (uint16_t *)D.adjT.p, stride, (int32_t *)D.flags.p, g0);
+    const int np = count * patched->max_patch;
+    hipLaunchKernelGGL(gg_patch_kernel, dim3((np + 255) / 256), dim3(256), 0, E.stream,
+                       patched->patches, patched->max_patch, count, rel_src[0], s.n, s.d, s.dp,
+                       (uint16_t *)D.adjT.p, stride, (int32_t *)D.flags.p);
   } else {
     // The per-graph descriptors were uploaded by upload_prep (outside any
     // capture); one launch covers every graph.
     (void)adj_src;
     (void)rel_src;
     (void)check_sym;
-    int max_n = 1;
-    for (const Slot &s : A.slots) max_n = std::max(max_n, s.n);
-    hipLaunchKernelGGL(gg_prep_kernel, dim3((max_n + 255) / 256, count), dim3(256), 0, E.stream,
-                       (const PrepDesc *)D.prep.p);
+    uint32_t max_entries = 1;
+    for (const Slot &s : A.slots) max_entries = std::max(max_entries, (uint32_t)s.n * (uint32_t)s.dp);
+    for (int g0 = 0; g0 < count; g0 += 65535)  // gridDim.y is at most 65535
+      hipLaunchKernelGGL(gg_prep_kernel, dim3((max_entries + 255) / 256, std::min(65535, count - g0)),
+                         dim3(256), 0, E.stream, (const PrepDesc *)D.prep.p + g0);
   }
   if (ev_begin) GG_HIP(hipEventRecord(ev_begin, E.stream));
   for (const LaunchGroup &lg : A.groups) {
