@@ -14,6 +14,8 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libggeval.so")
 GENERAL_PATH = os.path.join(PKG, "libgg_general.so")
 GRID_PATH = os.path.join(PKG, "libgg_grid.so")
+NONREGULAR_GENERAL_PATH = os.path.join(PKG, "libgg_nonregular_general.so")
+NONREGULAR_GRID_PATH = os.path.join(PKG, "libgg_nonregular_grid.so")
 
 GG_OK = 0
 GG_ERR_INVALID = -1

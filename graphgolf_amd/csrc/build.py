@@ -6,6 +6,8 @@ Outputs, next to the package so they travel with the tree:
     graphgolf_amd/libggeval.so     kernels + engine + host SA/graph code (C ABI: include/gg_eval.h)
     graphgolf_amd/libgg_general.so reference ABI shim: general evaluation()  (include/gg_general.h)
     graphgolf_amd/libgg_grid.so    reference ABI shim: grid evaluation()     (include/gg_grid.h)
+    graphgolf_amd/libgg_nonregular_{general,grid}.so  non-regular variants' evaluation()
+                                   (include/gg_nonregular.h)
     graphgolf_amd/gg_lds_probe     LDS random-gather throughput probe (executable)
 """
 from __future__ import annotations
@@ -69,9 +71,13 @@ def build(verbose: bool = False) -> None:
     core = os.path.join(PKG, "libggeval.so")
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", core + ".tmp", *objs])
     os.replace(core + ".tmp", core)
-    for shim, lib in (("gg_general_shim.cpp", "libgg_general.so"), ("gg_grid_shim.cpp", "libgg_grid.so")):
+    for shim, lib, defs in (("gg_general_shim.cpp", "libgg_general.so", []),
+                            ("gg_grid_shim.cpp", "libgg_grid.so", []),
+                            ("gg_nonregular_shim.cpp", "libgg_nonregular_general.so", []),
+                            ("gg_nonregular_shim.cpp", "libgg_nonregular_grid.so",
+                             ["-DGG_NONREGULAR_GRID"])):
         out = os.path.join(PKG, lib)
-        _run(["g++", *HOST_FLAGS, "-shared", "-o", out + ".tmp", os.path.join(HERE, shim),
+        _run(["g++", *HOST_FLAGS, *defs, "-shared", "-o", out + ".tmp", os.path.join(HERE, shim),
               "-L", PKG, "-lggeval", "-Wl,-rpath,$ORIGIN"])
         os.replace(out + ".tmp", out)
     # The LDS gather-throughput probe (the tile BFS's roofline on this GPU).

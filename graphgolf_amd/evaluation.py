@@ -244,3 +244,36 @@ def grid_evaluation(nodes: int, degree: int, groups: int, adjacency, based_nodes
     if r.connected and not (r.diameter > nodes):
         return True, r.diameter, r.sum / ((float(nodes) - 1) * nodes)
     return False, (r.max_ecc if enable_bfs else nodes + 1), None
+
+
+def _rows_padded(adj, lens) -> np.ndarray:
+    """Row v keeps its first lens[v] entries; the unused slots get v itself
+    (what the non-regular shims hand to the device)."""
+    a = np.array(L.c32(adj), copy=True)
+    n, w = a.shape
+    ln = np.asarray(lens, dtype=np.int64)
+    if ln.shape != (n,) or (ln < 0).any() or (ln > w).any():
+        raise ValueError("each row length must be in [0, width]")
+    mask = np.arange(w)[None, :] >= ln[:, None]
+    a[mask] = np.broadcast_to(np.arange(n, dtype=np.int32)[:, None], (n, w))[mask]
+    return a
+
+
+def evaluation_nonregular(nodes: int, based_nodes: int, groups: int, lines: int, degree: int,
+                          adj, num_degrees, added_centers: int = 0, algo: int = MATRIX_OP):
+    """variants/not_regular_general_graph/evaluation.c:378-396: row v of adj
+    ([nodes][degree]) uses its first num_degrees[v] entries; results as
+    evaluation()."""
+    a = _rows_padded(L.c32(adj).reshape(nodes, degree), num_degrees)
+    return evaluation(nodes, based_nodes, groups, lines, degree, a, added_centers, algo)
+
+
+def grid_evaluation_nonregular(nodes: int, max_degree: int, degree, groups: int, adjacency,
+                               based_nodes: int, height: int, based_height: int,
+                               enable_bfs: bool, rotate_hash):
+    """variants/not_regular_grid_graph/evaluation.c:194-208: row v of
+    adjacency ([nodes][max_degree]) uses its first degree[v] entries; results
+    as grid_evaluation()."""
+    a = _rows_padded(L.c32(adjacency).reshape(nodes, max_degree), degree)
+    return grid_evaluation(nodes, max_degree, groups, a, based_nodes, height, based_height,
+                           enable_bfs, rotate_hash)

@@ -354,3 +354,59 @@ def ref_sa_grid(edges, width, height, degree, length, groups, seed, ncalcs, max_
                  ctypes.byref(tot), width, bw, height, bh, length, groups, _ip(rh),
                  bool(fixed_temp > 0 and not hill_climbing))
     return e, diam.value, aspl.value, int(steps)
+
+
+def nonregular_padded(adj, lens) -> np.ndarray:
+    """The non-regular variants read only the first lens[v] entries of row v
+    (variants/not_regular_general_graph/evaluation.c:180, :291;
+    variants/not_regular_grid_graph/evaluation.c:25, :50); filling the rest
+    with v itself gives a table the regular evaluators read the same way (a
+    self-loop adds nothing to a reach set)."""
+    a = np.array(adj, dtype=np.int32, copy=True)
+    n, w = a.shape
+    cols = np.arange(w)[None, :]
+    mask = cols >= np.asarray(lens, dtype=np.int64)[:, None]
+    a[mask] = np.broadcast_to(np.arange(n, dtype=np.int32)[:, None], (n, w))[mask]
+    return a
+
+
+def ref_nonregular_evaluation(adj, lens, based_nodes=None, groups=1, added_centers=0,
+                              algo=1) -> Eval:
+    """The reference's non-regular general evaluation()
+    (variants/not_regular_general_graph/evaluation.c:378)."""
+    L = ref_lib("nrgeneral")
+    adj = _c32(adj)
+    ln = _c32(lens)
+    n, d = adj.shape
+    b = n - added_centers if based_nodes is None else based_nodes
+    diam = ctypes.c_int(0)
+    aspl = ctypes.c_double(0)
+    L.evaluation.restype = ctypes.c_bool
+    L.evaluation.argtypes = [ctypes.c_int] * 5 + [ctypes.POINTER(ctypes.c_int)] * 2 + \
+        [ctypes.POINTER(ctypes.c_double), ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int]
+    ok = L.evaluation(n, b, groups, int(ln.sum()) // 2, d, _ip(adj), ctypes.byref(diam),
+                      ctypes.byref(aspl), added_centers, _ip(ln), algo)
+    S = round(aspl.value * n * (n - 1)) if ok else 0
+    return Eval(bool(ok), diam.value, aspl.value if ok else None, int(S), diam.value)
+
+
+def ref_nonregular_grid_evaluation(adj, lens, groups, based_nodes, height, based_height,
+                                   rotate_hash, enable_bfs=False) -> Eval:
+    """The reference's non-regular grid evaluation()
+    (variants/not_regular_grid_graph/evaluation.c:194)."""
+    L = ref_lib("nrgrid")
+    adj = _c32(adj)
+    ln = _c32(lens)
+    rh = _c32(rotate_hash)
+    n, d = adj.shape
+    diam = ctypes.c_int(-7)
+    aspl = ctypes.c_double(-7.0)
+    ip = ctypes.POINTER(ctypes.c_int)
+    L.evaluation.restype = ctypes.c_bool
+    L.evaluation.argtypes = [ctypes.c_int, ctypes.c_int, ip, ctypes.c_int, ip, ctypes.c_int,
+                             ctypes.c_int, ctypes.c_int, ip, ctypes.POINTER(ctypes.c_double),
+                             ctypes.c_bool, ip]
+    ok = L.evaluation(n, d, _ip(ln), groups, _ip(adj), based_nodes, height, based_height,
+                      ctypes.byref(diam), ctypes.byref(aspl), enable_bfs, _ip(rh))
+    S = round(aspl.value * n * (n - 1)) if ok else 0
+    return Eval(bool(ok), diam.value, aspl.value if ok else None, int(S), diam.value)

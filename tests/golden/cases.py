@@ -278,3 +278,79 @@ def grid_sa_cases():
         dict(id="gsa:w10h10d6r3:fixed", width=10, height=10, degree=6, length=3, groups=1,
              graph_seed=7, seed=15, ncalcs=1000, fixed_temp=20.0),
     ]
+
+
+GOLDEN_NONREGULAR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden_nonregular.csv")
+
+
+def _drop_edges(e: np.ndarray, k: int, rng: np.random.Generator) -> np.ndarray:
+    keep = np.ones(len(e), dtype=bool)
+    keep[rng.choice(len(e), size=k, replace=False)] = False
+    return e[keep]
+
+
+def nonregular_cases():
+    """Inputs of the non-regular variants' evaluation(): the mesh files,
+    seeded regular graphs with edges removed (general, incl. centres and an
+    isolated vertex), and seeded grids with edges removed (grid)."""
+    cases = []
+    for f in ["3x3", "4x4", "3x3x3", "2x2x2x2", "3x3x3x3", "4x4x4"]:
+        for algo in (0, 1):
+            cases.append(dict(id=f"nr:{f}:{algo}", kind="nr_general", file=f, algo=algo))
+    for (n, d, k, seed) in [(256, 8, 100, 1), (1000, 6, 300, 2), (4096, 16, 2000, 3)]:
+        for algo in (0, 1, 2):
+            cases.append(dict(id=f"nr:n{n}d{d}-{k}:{algo}", kind="nr_general", n=n, d=d, k=k,
+                              seed=seed, algo=algo))
+    # The variant's centre pass reads every slot of a row, unused ones too
+    # (variants/not_regular_general_graph/evaluation.c:224-229), which its own
+    # driver leaves uninitialised; with centres the unused slots hold the
+    # row's own id, the one filling for which that read is well defined.
+    cases.append(dict(id="nr:centres", kind="nr_general", n=300, d=6, k=80, seed=4, c=20, algo=1,
+                      pad="self"))
+    cases.append(dict(id="nr:isolated", kind="nr_general", n=128, d=4, k=0, seed=5, isolate=7,
+                      algo=1))
+    for (w, h, d, r, k, seed) in [(10, 10, 4, 2, 20, 6), (20, 20, 6, 3, 60, 7), (50, 50, 4, 2, 200, 8)]:
+        for bfs in (False, True):
+            cases.append(dict(id=f"nrg:w{w}h{h}d{d}r{r}-{k}:{int(bfs)}", kind="nr_grid", w=w, h=h,
+                              d=d, r=r, k=k, seed=seed, bfs=bfs))
+    cases.append(dict(id="nrg:isolated", kind="nr_grid", w=12, h=12, d=4, r=2, k=0, seed=9,
+                      isolate=30, bfs=False))
+    return cases
+
+
+def build_nonregular(case: dict):
+    """Returns (adj, lens, kwargs): adj [n][width] holds row v's lens[v]
+    neighbours first; its unused slots hold seeded arbitrary ids (the
+    variants never read them)."""
+    rng = np.random.default_rng(case.get("seed", 0))
+    if case["kind"] == "nr_general":
+        if "file" in case:
+            e = G.read_edges(os.path.join(DATA, case["file"]))
+            n = int(e.max()) + 1
+        else:
+            n, d = case["n"], case["d"]
+            e = _drop_edges(G.random_regular(n, d, case["seed"]), case["k"], rng)
+    else:
+        w, h = case["w"], case["h"]
+        n = w * h
+        e = _drop_edges(G.random_grid(w, h, case["d"], case["r"], case["seed"]), case["k"], rng)
+    if "isolate" in case:
+        v = case["isolate"]
+        e = e[(e[:, 0] != v) & (e[:, 1] != v)]
+    lens = np.bincount(e.ravel(), minlength=n).astype(np.int32)
+    width = int(max(lens.max(), 1))
+    if case.get("pad") == "self":
+        adj = np.repeat(np.arange(n, dtype=np.int32)[:, None], width, axis=1)
+    else:
+        adj = rng.integers(0, n, size=(n, width)).astype(np.int32)
+    fill = np.zeros(n, dtype=np.int64)
+    for a, b in e.tolist():
+        adj[a, fill[a]] = b
+        fill[a] += 1
+        adj[b, fill[b]] = a
+        fill[b] += 1
+    if case["kind"] == "nr_general":
+        c = case.get("c", 0)
+        return adj, lens, dict(based_nodes=n - c, groups=1, added_centers=c)
+    return adj, lens, dict(groups=1, based_nodes=n, height=case["h"], based_height=case["h"],
+                           rotate_hash=np.arange(n, dtype=np.int32))

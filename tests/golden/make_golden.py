@@ -27,8 +27,9 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
 
 import oracle  # noqa: E402
-from cases import (GOLDEN_CSV, GOLDEN_GRID_SA, GOLDEN_SA, build, general_cases,  # noqa: E402
-                   grid_cases, grid_sa_cases, sa_cases)
+from cases import (GOLDEN_CSV, GOLDEN_GRID_SA, GOLDEN_NONREGULAR, GOLDEN_SA,  # noqa: E402
+                   build, build_nonregular, general_cases, grid_cases, grid_sa_cases,
+                   nonregular_cases, sa_cases)
 from graphgolf_amd import graphs as G  # noqa: E402
 
 
@@ -50,11 +51,35 @@ def grid_sa() -> None:
         json.dump(rows, f, indent=1)
 
 
+def nonregular() -> None:
+    """Outputs of the non-regular variants' own evaluation()."""
+    rows = []
+    for case in nonregular_cases():
+        adj, lens, kw = build_nonregular(case)
+        if case["kind"] == "nr_general":
+            r = oracle.ref_nonregular_evaluation(adj, lens, kw["based_nodes"], kw["groups"],
+                                                 kw["added_centers"], case["algo"])
+        else:
+            r = oracle.ref_nonregular_grid_evaluation(adj, lens, kw["groups"], kw["based_nodes"],
+                                                      kw["height"], kw["based_height"],
+                                                      kw["rotate_hash"], case["bfs"])
+        aspl = float(r.aspl).hex() if r.ok else ""
+        rows.append((case["id"], int(r.ok), r.diameter, aspl, r.S if r.ok else 0))
+        print(rows[-1], flush=True)
+    with open(GOLDEN_NONREGULAR, "w", newline="") as f:
+        w = csv.writer(f)
+        w.writerow(["id", "ok", "diameter", "aspl_hex", "S"])
+        w.writerows(rows)
+
+
 def main() -> None:
     if not (oracle.ref_available("general") and oracle.ref_available("grid")):
         raise SystemExit("oracle/_ref is not built (make -C oracle ref)")
     if "--grid-sa" in sys.argv:
         grid_sa()
+        return
+    if "--nonregular" in sys.argv:
+        nonregular()
         return
     rows = []
     for case in general_cases():
@@ -87,6 +112,7 @@ def main() -> None:
     with open(GOLDEN_SA, "w") as f:
         json.dump(sa_rows, f, indent=1)
     grid_sa()
+    nonregular()
 
 
 if __name__ == "__main__":
