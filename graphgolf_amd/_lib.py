@@ -18,6 +18,7 @@ NONREGULAR_GENERAL_PATH = os.path.join(PKG, "libgg_nonregular_general.so")
 NONREGULAR_GRID_PATH = os.path.join(PKG, "libgg_nonregular_grid.so")
 
 GG_OK = 0
+GG_ALREADY_OPTIMAL = 1  # *_program: the start graph already meets the lower bound
 GG_ERR_INVALID = -1
 GG_ERR_TOO_LARGE = -2
 GG_ERR_HIP = -3
@@ -120,6 +121,8 @@ def lib():
                                 ctypes.c_int),
         "gg_random_grid": ([ctypes.c_int] * 4 + [ctypes.c_uint64, i32p], ctypes.c_int),
         "gg_random_grid_symmetric": ([ctypes.c_int] * 5 + [ctypes.c_uint64, i32p], ctypes.c_int),
+        "gg_grid_symmetric_from_base": ([ctypes.c_int] * 5 + [ctypes.c_uint32, i32p, i32p],
+                                        ctypes.c_int),
         "gg_grid_lower_bound": ([ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_int),
                                                       ctypes.POINTER(ctypes.c_double)],
                                 ctypes.c_int),
@@ -127,6 +130,14 @@ def lib():
                         ctypes.POINTER(GGSaParams), ctypes.c_double, EVAL_GRID_FN,
                         ctypes.c_void_p, ctypes.POINTER(GGSaStats), i32p, u64p], ctypes.c_int),
         "gg_edges_to_adj": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, i32p, i32p], ctypes.c_int),
+        "gg_general_program": ([ctypes.c_int, ctypes.c_int, i32p, ctypes.c_int,
+                                ctypes.POINTER(GGSaParams), EVAL_FN, ctypes.c_void_p,
+                                ctypes.POINTER(GGSaStats), i32p, u64p], ctypes.c_int),
+        "gg_grid_lattice": ([ctypes.c_int] * 4 + [ctypes.c_uint32, i32p], ctypes.c_int),
+        "gg_grid_program": ([ctypes.POINTER(GGGridShape), ctypes.c_int, i32p, ctypes.c_int, i32p,
+                             ctypes.POINTER(GGSaParams), ctypes.c_double, EVAL_GRID_FN,
+                             ctypes.c_void_p, ctypes.POINTER(GGSaStats), i32p, u64p],
+                            ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

@@ -462,9 +462,12 @@ int device_eval(void *, int nodes, int degree, const int32_t *adj, int32_t *diam
 
 }  // namespace
 
-extern "C" int gg_sa_general(int nodes, int degree, int32_t *edges, const gg_sa_params *P,
-                             gg_eval_fn eval, void *ctx, gg_sa_stats *stats, int32_t *log_diam,
-                             uint64_t *log_sum) {
+namespace {
+// sa() (general/sa.c:413-520); seed_rng = false continues the libc random()
+// stream where the caller left it.
+int sa_general_run(int nodes, int degree, int32_t *edges, const gg_sa_params *P, gg_eval_fn eval,
+                   void *ctx, gg_sa_stats *stats, int32_t *log_diam, uint64_t *log_sum,
+                   bool seed_rng) {
   if (!edges || !P || nodes < 3 || degree < 1 || ((int64_t)nodes * degree) % 2)
     return set_error(GG_ERR_INVALID, "bad SA arguments");
   if (!eval) eval = device_eval;
@@ -502,7 +505,7 @@ extern "C" int gg_sa_general(int nodes, int degree, int32_t *edges, const gg_sa_
                       : (max_t != min_t) ? std::pow(min_t / max_t, (double)cycle / P->ncalcs)
                                          : 1.0;  // general/main.c:436-442
   double temp = max_t;
-  srandom(P->seed);
+  if (seed_rng) srandom(P->seed);
   int cur_diam, best_diam;
   double cur_aspl, best_aspl;
   int rc = evaluate(&cur_diam, &cur_aspl);
@@ -559,6 +562,97 @@ extern "C" int gg_sa_general(int nodes, int degree, int32_t *edges, const gg_sa_
     stats->seconds_eval = t_eval;
   }
   return GG_OK;
+}
+
+// A simple d-regular edge list: endpoints in range, full degree everywhere,
+// no self-loop, no repeated edge.
+int check_simple_regular(int nodes, int degree, const int32_t *edges) {
+  const int lines = nodes * degree / 2;
+  std::vector<int> deg(nodes, 0);
+  EdgeSet set((size_t)lines);
+  for (int i = 0; i < lines; ++i) {
+    const int a = edges[2 * i], b = edges[2 * i + 1];
+    if (a < 0 || a >= nodes || b < 0 || b >= nodes)
+      return set_error(GG_ERR_INVALID, "edge %d endpoint out of range", i);
+    if (a == b) return set_error(GG_ERR_INVALID, "edge %d is a self-loop", i);
+    if (set.has(ekey(a, b))) return set_error(GG_ERR_INVALID, "edge %d repeats an earlier edge", i);
+    set.add(ekey(a, b));
+    ++deg[a];
+    ++deg[b];
+  }
+  for (int v = 0; v < nodes; ++v)
+    if (deg[v] != degree)
+      return set_error(GG_ERR_INVALID, "vertex %d has degree %d, not %d", v, deg[v], degree);
+  return GG_OK;
+}
+
+}  // namespace
+
+extern "C" int gg_sa_general(int nodes, int degree, int32_t *edges, const gg_sa_params *P,
+                             gg_eval_fn eval, void *ctx, gg_sa_stats *stats, int32_t *log_diam,
+                             uint64_t *log_sum) {
+  return sa_general_run(nodes, degree, edges, P, eval, ctx, stats, log_diam, log_sum, true);
+}
+
+extern "C" int gg_general_program(int nodes, int degree, int32_t *edges, int halfway,
+                                  const gg_sa_params *P, gg_eval_fn eval, void *ctx,
+                                  gg_sa_stats *stats, int32_t *log_diam, uint64_t *log_sum) {
+  if (!edges || !P || nodes < 3 || degree < 1 || nodes <= degree || ((int64_t)nodes * degree) % 2)
+    return set_error(GG_ERR_INVALID, "bad program arguments (nodes %d degree %d)", nodes, degree);
+  int rc = check_simple_regular(nodes, degree, edges);
+  if (rc) return rc;
+  if (!eval) eval = device_eval;
+  const int lines = nodes * degree / 2;
+  auto connected = [&](const int32_t *adj, int *diam, double *aspl) -> int {
+    int32_t dm = 0, conn = 0;
+    uint64_t sum = 0;
+    const int r = eval(ctx, nodes, degree, adj, &dm, &sum, &conn);
+    if (r) return r;
+    const bool ok = conn && dm < nodes;  // general/evaluation.c:254
+    if (diam) *diam = dm;
+    if (aspl) *aspl = ok ? (double)sum / (((double)nodes - 1) * nodes) : 0.0;
+    return ok ? 1 : 0;
+  };
+  srandom(P->seed);  // general/main.c:360
+  Move mv{};
+  if (!halfway) {  // create_symmetric_edge with groups = 1 (general/main.c:175-210)
+    GeneralState S(nodes, degree, edges);
+    for (int i = 0; i < 100; ++i)
+      if ((rc = S.propose(mv))) return rc;
+    while (true) {
+      if ((rc = S.propose(mv))) return rc;
+      if ((rc = connected(S.adj.data(), nullptr, nullptr)) < 0) return rc;
+      if (rc == 1) break;
+    }
+  }
+  {  // check_current_edge (general/sa.c:557-572)
+    std::vector<int32_t> adj((size_t)nodes * degree);
+    if ((rc = gg_edges_to_adj(nodes, lines, degree, edges, adj.data()))) return rc;
+    int diam;
+    double aspl;
+    if ((rc = connected(adj.data(), &diam, &aspl)) < 0) return rc;
+    if (rc == 0)
+      return set_error(GG_ERR_INVALID, "The input file has a node which is never reached by another node.");
+    int low_diam;
+    double low_aspl;
+    moore(nodes, degree, &low_diam, &low_aspl);
+    if (aspl == low_aspl) {
+      if (stats) {
+        *stats = gg_sa_stats{};
+        stats->best_diameter = diam;
+        stats->best_aspl = aspl;
+      }
+      return GG_ALREADY_OPTIMAL;
+    }
+  }
+  {  // estimate_elapse_time's five moves on a copy (general/sa.c:523-554):
+     // only their random() draws carry over; their timing evaluations are skipped.
+    std::vector<int32_t> copy(edges, edges + 2 * (size_t)lines);
+    GeneralState T(nodes, degree, copy.data());
+    for (int i = 0; i < 5; ++i)
+      if ((rc = T.propose(mv))) return rc;
+  }
+  return sa_general_run(nodes, degree, edges, P, eval, ctx, stats, log_diam, log_sum, false);
 }
 
 
@@ -781,10 +875,12 @@ extern "C" int gg_grid_lower_bound(int width, int height, int degree, int length
   return GG_OK;
 }
 
-extern "C" int gg_sa_grid(const gg_grid_shape *shape, int degree, int32_t *edges,
-                          const int32_t *rotate_hash, const gg_sa_params *P, double fixed_temp,
-                          gg_eval_grid_fn eval, void *ctx, gg_sa_stats *stats, int32_t *log_diam,
-                          uint64_t *log_sum) {
+namespace {
+// sa() (grid/sa.c:283-361); seed_rng = false continues the libc random()
+// stream where the caller left it.
+int sa_grid_run(const gg_grid_shape *shape, int degree, int32_t *edges, const int32_t *rotate_hash,
+                const gg_sa_params *P, double fixed_temp, gg_eval_grid_fn eval, void *ctx,
+                gg_sa_stats *stats, int32_t *log_diam, uint64_t *log_sum, bool seed_rng) {
   if (!shape || !edges || !P || degree < 1)
     return set_error(GG_ERR_INVALID, "bad grid SA arguments");
   const int width = shape->width, height = shape->height, groups = shape->groups;
@@ -842,7 +938,7 @@ extern "C" int gg_sa_grid(const gg_grid_shape *shape, int degree, int32_t *edges
   const int cycle = P->cooling_cycle > 0 ? P->cooling_cycle : 1;
   const double rate = P->hill_climbing ? 1.0 : std::pow(min_t / max_t, (double)cycle / P->ncalcs);
   double temp = max_t;
-  srandom(P->seed);
+  if (seed_rng) srandom(P->seed);
   int cur_diam, best_diam;
   double cur_aspl, best_aspl;
   rc = evaluate(edges, &cur_diam, &cur_aspl);
@@ -895,6 +991,15 @@ extern "C" int gg_sa_grid(const gg_grid_shape *shape, int degree, int32_t *edges
   }
   return GG_OK;
 }
+}  // namespace
+
+extern "C" int gg_sa_grid(const gg_grid_shape *shape, int degree, int32_t *edges,
+                          const int32_t *rotate_hash, const gg_sa_params *P, double fixed_temp,
+                          gg_eval_grid_fn eval, void *ctx, gg_sa_stats *stats, int32_t *log_diam,
+                          uint64_t *log_sum) {
+  return sa_grid_run(shape, degree, edges, rotate_hash, P, fixed_temp, eval, ctx, stats, log_diam,
+                     log_sum, true);
+}
 
 namespace {
 // Vertices not reached from vertex 0 (simple_bfs, grid/main.c:197-230).
@@ -929,6 +1034,28 @@ extern "C" int gg_random_grid_symmetric(int width, int height, int degree, int l
   std::vector<int32_t> base((size_t)2 * bl);
   int rc = gg_random_grid(bw, bh, degree, length, seed, base.data());
   if (rc) return rc;
+  return gg_grid_symmetric_from_base(width, height, degree, length, groups, (uint32_t)seed,
+                                     base.data(), edges);
+}
+
+namespace {
+// create_symmetric_edge (grid/main.c:393-445) on the current libc random()
+// stream; base may alias the first lines/groups edges of `edges`.
+int symmetric_from_base_run(int width, int height, int degree, int length, int groups,
+                            const int32_t *base_in, int32_t *edges) {
+  if (!edges || (groups != 2 && groups != 4))
+    return set_error(GG_ERR_INVALID, "groups must be 2 or 4");
+  if ((groups == 4 && (width != height || width % 2)) || height % 2)
+    return set_error(GG_ERR_INVALID, "grid %dx%d cannot carry %d-fold symmetry", width, height, groups);
+  const int bw = groups == 4 ? width / 2 : width, bh = height / 2;
+  const int nodes = width * height, lines = nodes * degree / 2, bl = lines / groups;
+  if (((int64_t)nodes * degree) % 2 || lines % groups || length < 1)
+    return set_error(GG_ERR_INVALID, "nodes*degree/2 must be a whole multiple of groups");
+  if (!base_in) return set_error(GG_ERR_INVALID, "no base edges");
+  const std::vector<int32_t> base(base_in, base_in + 2 * (size_t)bl);
+  for (int i = 0; i < 2 * bl; ++i)
+    if (base[i] < 0 || base[i] >= bw * bh)
+      return set_error(GG_ERR_INVALID, "base edge endpoint %d outside the %dx%d base grid", base[i], bw, bh);
   GridGeom G{height, width, groups};
   for (int i = 0; i < 2 * bl; ++i) edges[i] = (base[i] / bh) * height + base[i] % bh;
   for (int i = 0; i < bl; ++i)
@@ -942,8 +1069,8 @@ extern "C" int gg_random_grid_symmetric(int width, int height, int degree, int l
         edges[2 * (3 * bl + i) + j] = G.rotate(v, 270);
       }
     }
+  int rc = GG_OK;
   GridMover M{lines, bl, G, length};
-  srandom((unsigned)seed);
   for (int64_t i = 0; i < (int64_t)lines * 10; ++i) M.exchange(edges);  // INITIAL_TIMES
   std::vector<int32_t> adj((size_t)nodes * degree), tmp(edges, edges + 2 * (size_t)lines);
   if ((rc = gg_edges_to_adj(nodes, lines, degree, edges, adj.data()))) return rc;
@@ -960,4 +1087,213 @@ extern "C" int gg_random_grid_symmetric(int width, int height, int degree, int l
     }
   }
   return GG_OK;
+}
+
+// create_lattice (grid/main.c:232-330) on a width x height grid: `degree`
+// parallel edges between columns 2x and 2x+1 (an odd last column pairs rows
+// 2y and 2y+1; an odd last vertex takes degree/2 self-loops), lines*10
+// length-limited swaps (simple_exchange_edge, grid/main.c:120-143), then swaps
+// on a copy kept while they do not raise the number of vertices unreached
+// from vertex 0 (always at least one swap, until none is unreached), then of
+// self-loops, then of repeated edges.  Draws random() % lines as the
+// reference does.
+int lattice_run(int width, int height, int degree, int length, int32_t *e) {
+  const int nodes = width * height, lines = nodes * degree / 2;
+  int i = 0;
+  for (int x = 0; x < width / 2; ++x)
+    for (int y = 0; y < height; ++y)
+      for (int k = 0; k < degree; ++k, ++i) {
+        e[2 * i] = y + 2 * x * height;
+        e[2 * i + 1] = e[2 * i] + height;
+      }
+  if (width % 2) {
+    for (int y = 0; y < height / 2; ++y)
+      for (int k = 0; k < degree; ++k, ++i) {
+        e[2 * i] = (width - 1) * height + 2 * y;
+        e[2 * i + 1] = e[2 * i] + 1;
+      }
+    if (height % 2)
+      for (int k = 0; k < degree / 2; ++k, ++i) e[2 * i] = e[2 * i + 1] = nodes - 1;
+  }
+  if (i != lines) return set_error(GG_ERR_INVALID, "a %dx%d lattice of degree %d does not close", width, height, degree);
+  GridGeom G{height, width, 1};
+  auto swap = [&](int32_t *f) {
+    while (true) {
+      int e1, e2;
+      do {
+        e1 = (int)(random() % lines);
+        e2 = (int)(random() % lines);
+      } while (e1 == e2);
+      const int v1 = f[2 * e1], w1 = f[2 * e1 + 1], v2 = f[2 * e2], w2 = f[2 * e2 + 1];
+      if (G.dist(v1, v2) <= length && G.dist(w1, w2) <= length) {
+        f[2 * e1 + 1] = v2;
+        f[2 * e2] = w1;
+      } else if (G.dist(v1, w2) <= length && G.dist(w1, v2) <= length) {
+        f[2 * e1 + 1] = w2;
+        f[2 * e2] = w1;
+        f[2 * e2 + 1] = v2;
+      } else {
+        continue;
+      }
+      return;
+    }
+  };
+  for (int64_t t = 0; t < (int64_t)lines * 10; ++t) swap(e);  // INITIAL_TIMES
+  std::vector<int32_t> adj((size_t)nodes * degree), tmp(e, e + 2 * (size_t)lines);
+  int rc;
+  auto unreached = [&](const int32_t *f, int *num) -> int {
+    int r = gg_edges_to_adj(nodes, lines, degree, f, adj.data());
+    if (r) return r;
+    *num = unreached_from_zero(nodes, degree, adj);
+    return GG_OK;
+  };
+  auto loops = [&](const int32_t *f, int *num) -> int {
+    int c = 0;
+    for (int l = 0; l < lines; ++l) c += f[2 * l] == f[2 * l + 1];
+    *num = c;
+    return GG_OK;
+  };
+  auto repeats = [&](const int32_t *f, int *num) -> int {  // pairs of equal lines
+    std::vector<uint64_t> keys;
+    keys.reserve(lines);
+    for (int l = 0; l < lines; ++l) keys.push_back(ekey(f[2 * l], f[2 * l + 1]));
+    std::sort(keys.begin(), keys.end());
+    long long c = 0;
+    for (size_t a = 0; a < keys.size();) {
+      size_t b = a;
+      while (b < keys.size() && keys[b] == keys[a]) ++b;
+      c += (long long)(b - a) * (b - a - 1) / 2;
+      a = b;
+    }
+    *num = (int)std::min<long long>(c, 1 << 30);
+    return GG_OK;
+  };
+  // The connectivity pass runs at least once; the other two only when needed.
+  auto reduce = [&](auto count, bool always) -> int {
+    int min_num;
+    int r = count(e, &min_num);
+    if (r) return r;
+    if (!always && min_num == 0) return GG_OK;
+    for (int64_t tries = 0;; ++tries) {
+      if (tries > 10000000) return set_error(GG_ERR_INVALID, "lattice construction did not converge");
+      std::copy(e, e + 2 * (size_t)lines, tmp.begin());
+      swap(tmp.data());
+      int num;
+      if ((r = count(tmp.data(), &num))) return r;
+      if (num == 0) {
+        std::copy(tmp.begin(), tmp.end(), e);
+        return GG_OK;
+      }
+      if (num <= min_num) {
+        min_num = num;
+        std::copy(tmp.begin(), tmp.end(), e);
+      }
+    }
+  };
+  if ((rc = reduce(unreached, true))) return rc;
+  if ((rc = reduce(loops, false))) return rc;
+  return reduce(repeats, false);
+}
+}  // namespace
+
+extern "C" int gg_grid_symmetric_from_base(int width, int height, int degree, int length,
+                                           int groups, uint32_t seed, const int32_t *base,
+                                           int32_t *edges) {
+  srandom(seed);
+  return symmetric_from_base_run(width, height, degree, length, groups, base, edges);
+}
+
+extern "C" int gg_grid_lattice(int width, int height, int degree, int length, uint32_t seed,
+                               int32_t *edges) {
+  if (!edges || width < 1 || height < 1 || degree < 1 || length < 1 ||
+      ((int64_t)width * height * degree) % 2)
+    return set_error(GG_ERR_INVALID, "bad lattice arguments");
+  srandom(seed);
+  return lattice_run(width, height, degree, length, edges);
+}
+
+extern "C" int gg_grid_program(const gg_grid_shape *shape, int degree, int32_t *edges, int source,
+                               const int32_t *rotate_hash, const gg_sa_params *P, double fixed_temp,
+                               gg_eval_grid_fn eval, void *ctx, gg_sa_stats *stats,
+                               int32_t *log_diam, uint64_t *log_sum) {
+  if (!shape || !edges || !P || degree < 1 || source < GG_GRID_LATTICE || source > GG_GRID_FULL_FILE)
+    return set_error(GG_ERR_INVALID, "bad grid program arguments");
+  const int width = shape->width, height = shape->height, g = shape->groups, length = shape->length;
+  if (width < 1 || height < 1 || (g != 1 && g != 2 && g != 4) || length < 1)
+    return set_error(GG_ERR_INVALID, "bad grid shape %dx%d g=%d r=%d", width, height, g, length);
+  const int bw = g == 4 ? width / 2 : width, bh = g == 1 ? height : height / 2;
+  const int nodes = width * height;
+  // grid/main.c:684-695
+  if (g == 4 && bw != bh)
+    return set_error(GG_ERR_INVALID, "When g = 4, width(%d) must be equal to height(%d).", bw, bh);
+  if (g == 4 && width % 2 && height % 2)
+    return set_error(GG_ERR_INVALID, "When g = 4, width(%d) and height(%d) are divisible by 2.", width, height);
+  if (g == 2 && height % 2)
+    return set_error(GG_ERR_INVALID, "When g = 2, height(%d) os divisible by 2.", height);
+  if (((int64_t)nodes * degree) % 2 || nodes % g || (nodes * degree / 2) % g)
+    return set_error(GG_ERR_INVALID, "(nodes*degree/2) must be divisible by groups(%d)", g);
+  const int lines = nodes * degree / 2;
+  if (!eval) eval = device_eval_grid;
+  std::vector<int32_t> ident;
+  if (!rotate_hash) {
+    if (g != 1) return set_error(GG_ERR_INVALID, "groups > 1 needs rotate_hash");
+    ident.resize(nodes);
+    std::iota(ident.begin(), ident.end(), 0);
+    rotate_hash = ident.data();
+  }
+  srandom(P->seed);  // grid/main.c:630
+  int rc;
+  if (source == GG_GRID_LATTICE && (rc = lattice_run(bw, bh, degree, length, edges))) return rc;
+  if (source != GG_GRID_FULL_FILE && g != 1 &&
+      (rc = symmetric_from_base_run(width, height, degree, length, g, edges, edges)))
+    return rc;
+  // verfy_graph (grid/main.c:446-472)
+  std::vector<int> deg(nodes, 0);
+  GridGeom G{height, width, g};
+  for (int i = 0; i < lines; ++i) {
+    const int a = edges[2 * i], b = edges[2 * i + 1];
+    if (a < 0 || a >= nodes || b < 0 || b >= nodes)
+      return set_error(GG_ERR_INVALID, "edge %d endpoint out of range", i);
+    ++deg[a];
+    ++deg[b];
+    if (G.dist(a, b) > length)
+      return set_error(GG_ERR_INVALID, "Over length in line %d: length = %d, distance = %d", i + 1,
+                       length, G.dist(a, b));
+  }
+  for (int v = 0; v < nodes; ++v)
+    if (deg[v] != degree)
+      return set_error(GG_ERR_INVALID, "Not regular graph. degree = %d n[%d] = %d.", degree, v, deg[v]);
+  {  // check_current_edge (grid/sa.c:378-398)
+    std::vector<int32_t> adj((size_t)nodes * degree);
+    if ((rc = gg_edges_to_adj(nodes, lines, degree, edges, adj.data()))) return rc;
+    int32_t dm = 0, conn = 0;
+    uint64_t sum = 0;
+    if ((rc = eval(ctx, nodes, degree, g, nodes / g, adj.data(), rotate_hash, &dm, &sum, &conn)))
+      return rc;
+    if (!conn || dm > nodes)
+      return set_error(GG_ERR_INVALID, "The input file has a node which is never reached by another node.");
+    const double aspl = (double)sum / (((double)nodes - 1) * nodes);
+    int low_diam;
+    double low_aspl;
+    if ((rc = gg_grid_lower_bound(width, height, degree, length, &low_diam, &low_aspl))) return rc;
+    if (dm == low_diam && aspl == low_aspl) {
+      if (stats) {
+        *stats = gg_sa_stats{};
+        stats->best_diameter = dm;
+        stats->best_aspl = aspl;
+      }
+      return GG_ALREADY_OPTIMAL;
+    }
+  }
+  {  // estimated_elapse_time's five moves, each on a fresh copy (grid/sa.c:350-376):
+     // only their random() draws carry over; their timing evaluations are skipped.
+    GridMover M{lines, lines / g, G, length};
+    std::vector<int32_t> tmp(2 * (size_t)lines);
+    for (int i = 0; i < 5; ++i) {
+      std::copy(edges, edges + 2 * (size_t)lines, tmp.begin());
+      M.exchange(tmp.data());
+    }
+  }
+  return sa_grid_run(shape, degree, edges, rotate_hash, P, fixed_temp, eval, ctx, stats, log_diam,
+                     log_sum, false);
 }

@@ -47,6 +47,28 @@ def random_grid_symmetric(width: int, height: int, degree: int, length: int, gro
     return e
 
 
+def grid_symmetric_from_base(width: int, height: int, degree: int, length: int, groups: int,
+                             seed: int, base_edges) -> np.ndarray:
+    """create_symmetric_edge (grid/main.c:393-445) from a base-grid edge list
+    (vertex v at (v // base_height, v % base_height))."""
+    n = width * height
+    e = np.zeros((n * degree // 2, 2), dtype=np.int32)
+    b = L.c32(base_edges).reshape(-1, 2)
+    if b.shape[0] * groups != e.shape[0]:
+        raise ValueError(f"{b.shape[0]} base lines x {groups} groups != {e.shape[0]} lines")
+    L.check(L.lib().gg_grid_symmetric_from_base(width, height, degree, length, groups, seed,
+                                                L.ptr(b), L.ptr(e)))
+    return e
+
+
+def grid_lattice(width: int, height: int, degree: int, length: int, seed: int) -> np.ndarray:
+    """The reference's random lattice (create_lattice, grid/main.c:232-330)
+    after srandom(seed)."""
+    e = np.zeros((width * height * degree // 2, 2), dtype=np.int32)
+    L.check(L.lib().gg_grid_lattice(width, height, degree, length, seed, L.ptr(e)))
+    return e
+
+
 def grid_lower_bound(width: int, height: int, degree: int, length: int):
     """(diameter, ASPL) lower bounds of a grid graph (grid/main.c:480-529)."""
     d = ctypes.c_int(0)

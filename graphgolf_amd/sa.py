@@ -28,6 +28,53 @@ class SaResult:
     seconds_eval: float
     log_diameter: Optional[np.ndarray] = None
     log_sum: Optional[np.ndarray] = None
+    already_optimal: bool = False
+
+
+def _general_callback(evaluator):
+    """ctypes evaluator callback (None: the device evaluator)."""
+    if evaluator is None:
+        return ctypes.cast(None, L.EVAL_FN)
+
+    def _cb(ctx, n, d, adj_p, diam_p, sum_p, conn_p):
+        try:
+            adj = np.ctypeslib.as_array(adj_p, shape=(n, d)).copy()
+            ok, dm, S = evaluator(adj)
+            diam_p[0] = int(dm)
+            sum_p[0] = int(S)
+            conn_p[0] = int(bool(ok))
+            return 0
+        except Exception:  # surfaced as an error code
+            return -1
+    return L.EVAL_FN(_cb)
+
+
+def _grid_callback(evaluator):
+    if evaluator is None:
+        return ctypes.cast(None, L.EVAL_GRID_FN)
+
+    def _cb(ctx, n, d, g, b, adj_p, rh_p, diam_p, sum_p, conn_p):
+        try:
+            adj = np.ctypeslib.as_array(adj_p, shape=(n, d)).copy()
+            rhash = np.ctypeslib.as_array(rh_p, shape=(n,)).copy()
+            ok, dm, S = evaluator(adj, g, b, rhash)
+            diam_p[0] = int(dm)
+            sum_p[0] = int(S)
+            conn_p[0] = int(bool(ok))
+            return 0
+        except Exception:  # surfaced as an error code
+            return -1
+    return L.EVAL_GRID_FN(_cb)
+
+
+def _result(rc, e, st, log, ld, ls) -> SaResult:
+    if rc != L.GG_ALREADY_OPTIMAL:
+        L.check(rc)
+    n_log = min(log, st.evaluations)
+    return SaResult(e, st.steps, st.accepts, st.evaluations, st.best_diameter, st.best_aspl,
+                    st.seconds_total, st.seconds_eval,
+                    ld[:n_log] if log else None, ls[:n_log] if log else None,
+                    already_optimal=rc == L.GG_ALREADY_OPTIMAL)
 
 
 def sa_general(edges, nodes: int, degree: int, seed: int, ncalcs: int, max_temp: float = 100.0,
@@ -40,29 +87,28 @@ def sa_general(edges, nodes: int, degree: int, seed: int, ncalcs: int, max_temp:
     st = L.GGSaStats()
     ld = np.zeros(max(log, 1), dtype=np.int32)
     ls = np.zeros(max(log, 1), dtype=np.uint64)
-    cb = ctypes.cast(None, L.EVAL_FN)
-    keep = None
-    if evaluator is not None:
-        def _cb(ctx, n, d, adj_p, diam_p, sum_p, conn_p):
-            try:
-                adj = np.ctypeslib.as_array(adj_p, shape=(n, d)).copy()
-                ok, dm, S = evaluator(adj)
-                diam_p[0] = int(dm)
-                sum_p[0] = int(S)
-                conn_p[0] = int(bool(ok))
-                return 0
-            except Exception:  # surfaced as an error code
-                return -1
-        keep = L.EVAL_FN(_cb)
-        cb = keep
+    cb = _general_callback(evaluator)
     rc = L.lib().gg_sa_general(nodes, degree, L.ptr(e), ctypes.byref(p), cb, None,
-                               ctypes.byref(st), L.ptr(ld),
-                               ls.ctypes.data_as(L.u64p))
-    L.check(rc)
-    n_log = min(log, st.evaluations)
-    return SaResult(e, st.steps, st.accepts, st.evaluations, st.best_diameter, st.best_aspl,
-                    st.seconds_total, st.seconds_eval,
-                    ld[:n_log] if log else None, ls[:n_log] if log else None)
+                               ctypes.byref(st), L.ptr(ld), ls.ctypes.data_as(L.u64p))
+    return _result(rc, e, st, log, ld, ls)
+
+
+def general_program(edges, nodes: int, degree: int, seed: int, ncalcs: int, halfway: bool = False,
+                    max_temp: float = 100.0, min_temp: float = 0.217147, hill_climbing: bool = False,
+                    evaluator: Optional[Callable] = None, log: int = 0) -> SaResult:
+    """The general program's run from its start graph (general/main.c:360-462):
+    the pre-run moves unless `halfway` (-M), the start-graph check, then SA, all
+    on one random() stream seeded with `seed`.  `already_optimal` is set when
+    the start graph meets the Moore bound (nothing run)."""
+    e = L.c32(edges).reshape(-1, 2).copy()
+    p = L.GGSaParams(ncalcs, max_temp, min_temp, 1, int(hill_climbing), seed, log)
+    st = L.GGSaStats()
+    ld = np.zeros(max(log, 1), dtype=np.int32)
+    ls = np.zeros(max(log, 1), dtype=np.uint64)
+    cb = _general_callback(evaluator)
+    rc = L.lib().gg_general_program(nodes, degree, L.ptr(e), int(halfway), ctypes.byref(p), cb,
+                                    None, ctypes.byref(st), L.ptr(ld), ls.ctypes.data_as(L.u64p))
+    return _result(rc, e, st, log, ld, ls)
 
 
 def sa_grid(edges, width: int, height: int, degree: int, length: int, seed: int, ncalcs: int,
@@ -82,30 +128,43 @@ def sa_grid(edges, width: int, height: int, degree: int, length: int, seed: int,
     st = L.GGSaStats()
     ld = np.zeros(max(log, 1), dtype=np.int32)
     ls = np.zeros(max(log, 1), dtype=np.uint64)
-    cb = ctypes.cast(None, L.EVAL_GRID_FN)
-    keep = None
-    if evaluator is not None:
-        def _cb(ctx, n, d, g, b, adj_p, rh_p, diam_p, sum_p, conn_p):
-            try:
-                adj = np.ctypeslib.as_array(adj_p, shape=(n, d)).copy()
-                rhash = np.ctypeslib.as_array(rh_p, shape=(n,)).copy()
-                ok, dm, S = evaluator(adj, g, b, rhash)
-                diam_p[0] = int(dm)
-                sum_p[0] = int(S)
-                conn_p[0] = int(bool(ok))
-                return 0
-            except Exception:  # surfaced as an error code
-                return -1
-        keep = L.EVAL_GRID_FN(_cb)
-        cb = keep
+    cb = _grid_callback(evaluator)
     rc = L.lib().gg_sa_grid(ctypes.byref(shape), degree, L.ptr(e), L.ptr(rh), ctypes.byref(p),
                             fixed_temp, cb, None, ctypes.byref(st), L.ptr(ld),
                             ls.ctypes.data_as(L.u64p))
-    L.check(rc)
-    n_log = min(log, st.evaluations)
-    return SaResult(e, st.steps, st.accepts, st.evaluations, st.best_diameter, st.best_aspl,
-                    st.seconds_total, st.seconds_eval,
-                    ld[:n_log] if log else None, ls[:n_log] if log else None)
+    return _result(rc, e, st, log, ld, ls)
+
+
+GRID_LATTICE, GRID_BASE_FILE, GRID_FULL_FILE = 0, 1, 2
+
+
+def grid_program(edges, width: int, height: int, degree: int, length: int, seed: int, ncalcs: int,
+                 source: int, groups: int = 1, max_temp: float = 100.0, min_temp: float = 0.217147,
+                 cooling_cycle: int = 1, hill_climbing: bool = False, fixed_temp: float = 0.0,
+                 evaluator: Optional[Callable] = None, log: int = 0) -> SaResult:
+    """The grid program's run (grid/main.c:630-727): the lattice (source
+    GRID_LATTICE, `edges` ignored), the symmetric copies (GRID_BASE_FILE with
+    groups > 1: `edges` holds the base grid's lines in base coordinates), the
+    checks, then SA, all on one random() stream seeded with `seed`."""
+    from .graphs import rotate_hash as make_rotate_hash
+
+    nodes = width * height
+    lines = nodes * degree // 2
+    e = np.zeros((lines, 2), dtype=np.int32)
+    if source != GRID_LATTICE:
+        src = L.c32(edges).reshape(-1, 2)
+        e[:len(src)] = src
+    rh = make_rotate_hash(nodes, height, width, groups)
+    shape = L.GGGridShape(width, height, groups, length)
+    p = L.GGSaParams(ncalcs, max_temp, min_temp, cooling_cycle, int(hill_climbing), seed, log)
+    st = L.GGSaStats()
+    ld = np.zeros(max(log, 1), dtype=np.int32)
+    ls = np.zeros(max(log, 1), dtype=np.uint64)
+    cb = _grid_callback(evaluator)
+    rc = L.lib().gg_grid_program(ctypes.byref(shape), degree, L.ptr(e), source, L.ptr(rh),
+                                 ctypes.byref(p), fixed_temp, cb, None, ctypes.byref(st),
+                                 L.ptr(ld), ls.ctypes.data_as(L.u64p))
+    return _result(rc, e, st, log, ld, ls)
 
 
 class BsaParams(ctypes.Structure):

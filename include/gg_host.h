@@ -78,6 +78,25 @@ int gg_sa_general(int nodes, int degree, int32_t *edges, const gg_sa_params *par
                   gg_eval_fn eval, void *eval_ctx, gg_sa_stats *stats, int32_t *log_diam,
                   uint64_t *log_sum);
 
+/* Returned by the *_program entry points when the start graph already meets
+ * the lower bound: nothing is run, stats hold the start graph's values
+ * ("The input file has already optimum solution.", general/sa.c:568-569). */
+#define GG_ALREADY_OPTIMAL 1
+
+/* The general program's whole run after its argument checks, groups = 1
+ * (general/main.c:360-462), on one libc random() stream seeded with
+ * params->seed: unless `halfway` (-M), 100 random 2-opt moves and then moves
+ * until the graph is connected (create_symmetric_edge, general/main.c:175-210);
+ * the start-graph check (check_current_edge, general/sa.c:557-572); the five
+ * timing moves on a copy (estimate_elapse_time, general/sa.c:523-554; their
+ * evaluations are skipped, they do not touch the stream); then sa() as
+ * gg_sa_general without re-seeding.  `edges` must be a simple d-regular graph
+ * (GG_ERR_INVALID otherwise); on return it holds the best graph, as the
+ * reference's -o file does. */
+int gg_general_program(int nodes, int degree, int32_t *edges, int halfway,
+                       const gg_sa_params *params, gg_eval_fn eval, void *eval_ctx,
+                       gg_sa_stats *stats, int32_t *log_diam, uint64_t *log_sum);
+
 /* ---- Grid SA (grid/sa.c:283-361), same RNG draws (libc random() after
  * srandom(seed)) and moves as the reference: length-limited 2-opt with
  * groups-fold rotational symmetry (exchange_edge, grid/sa.c:137-221;
@@ -101,6 +120,18 @@ typedef int (*gg_eval_grid_fn)(void *ctx, int nodes, int degree, int groups, int
 int gg_random_grid_symmetric(int width, int height, int degree, int length, int groups,
                              uint64_t seed, int32_t *edges);
 
+/* The same from a given base-grid edge list: base[] holds the lines/groups
+ * edges of the base sub-grid (width x height/2 for g = 2, width/2 x height/2
+ * for g = 4, vertex v at (v / base_height, v % base_height)), as read from a
+ * -f file (grid/main.c:393-445 with srandom(seed)). */
+int gg_grid_symmetric_from_base(int width, int height, int degree, int length, int groups,
+                                uint32_t seed, const int32_t *base, int32_t *edges);
+
+/* create_lattice (grid/main.c:232-330) after srandom(seed): the reference's
+ * random length-limited lattice on a width x height grid, vertex v at
+ * (v / height, v % height). */
+int gg_grid_lattice(int width, int height, int degree, int length, uint32_t seed, int32_t *edges);
+
 /* lower_bound_of_diam_aspl (grid/main.c:480-529). */
 int gg_grid_lower_bound(int width, int height, int degree, int length, int *low_diam,
                         double *low_aspl);
@@ -112,6 +143,26 @@ int gg_grid_lower_bound(int width, int height, int degree, int length, int *low_
 int gg_sa_grid(const gg_grid_shape *shape, int degree, int32_t *edges, const int32_t *rotate_hash,
                const gg_sa_params *params, double fixed_temp, gg_eval_grid_fn eval, void *eval_ctx,
                gg_sa_stats *stats, int32_t *log_diam, uint64_t *log_sum);
+
+/* The grid program's whole run after its argument checks (grid/main.c:630-727)
+ * on one libc random() stream seeded with params->seed.  source:
+ *   GG_GRID_LATTICE    no -f: create_lattice on the base grid, written to edges;
+ *   GG_GRID_BASE_FILE  -f without -M: edges[0 .. lines/groups) hold the base
+ *                      grid's edges (base coordinates, as read from the file);
+ *   GG_GRID_FULL_FILE  -f with -M: edges hold the whole graph.
+ * then, unless GG_GRID_FULL_FILE or groups = 1, create_symmetric_edge
+ * (grid/main.c:393-445); verfy_graph's degree and length checks; the
+ * start-graph check; the five timing moves, each on a fresh copy
+ * (estimated_elapse_time, grid/sa.c:350-376, evaluations skipped); then sa()
+ * as gg_sa_grid without re-seeding.  edges: [lines][2]; on return the best
+ * graph. */
+#define GG_GRID_LATTICE 0
+#define GG_GRID_BASE_FILE 1
+#define GG_GRID_FULL_FILE 2
+int gg_grid_program(const gg_grid_shape *shape, int degree, int32_t *edges, int source,
+                    const int32_t *rotate_hash, const gg_sa_params *params, double fixed_temp,
+                    gg_eval_grid_fn eval, void *eval_ctx, gg_sa_stats *stats, int32_t *log_diam,
+                    uint64_t *log_sum);
 
 /* ---- Batched, surrogate-ranked simulated annealing (no reference
  * counterpart; the accept rule is the reference's, general/sa.c:371-411).

@@ -354,3 +354,62 @@ def build_nonregular(case: dict):
         return adj, lens, dict(based_nodes=n - c, groups=1, added_centers=c)
     return adj, lens, dict(groups=1, based_nodes=n, height=case["h"], based_height=case["h"],
                            rotate_hash=np.arange(n, dtype=np.int32))
+
+
+# ---- Whole command-line programs (general/main.c, grid/main.c) ------------
+
+GOLDEN_PROGRAMS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden_programs.json")
+
+
+def program_cases():
+    """Runs of the reference's programs: `input` names the start file this
+    build writes (None: the grid program builds its own lattice), `args` the
+    flags after -f/-o."""
+    return [
+        dict(id="gen_n64d4", kind="general", input=("regular", 64, 4, 7), args="-s 3 -n 200"),
+        dict(id="gen_n64d4_M", kind="general", input=("regular", 64, 4, 7), args="-M -s 8 -n 150"),
+        dict(id="gen_n64d4_H", kind="general", input=("regular", 64, 4, 7), args="-H -s 8 -n 150"),
+        dict(id="gen_n200d5_wc", kind="general", input=("regular", 200, 5, 2),
+             args="-s 11 -n 300 -w 50 -c 0.5"),
+        dict(id="gen_n256d6", kind="general", input=("regular", 256, 6, 3), args="-s 2 -n 400"),
+        dict(id="grid_lat_6x6", kind="grid", input=None, args="-W 6 -H 6 -D 4 -R 3 -s 5 -n 100"),
+        dict(id="grid_lat_6x6_g2", kind="grid", input=None,
+             args="-W 6 -H 6 -D 4 -R 2 -g 2 -s 5 -n 100"),
+        dict(id="grid_lat_8x8_g4", kind="grid", input=None,
+             args="-W 8 -H 8 -D 4 -R 3 -g 4 -s 9 -n 100"),
+        dict(id="grid_lat_7x5_odd", kind="grid", input=None, args="-W 7 -H 5 -D 4 -R 2 -s 3 -n 100"),
+        dict(id="grid_lat_5x6_d3", kind="grid", input=None, args="-W 5 -H 6 -D 3 -R 2 -s 2 -n 100"),
+        dict(id="grid_base_g2", kind="grid", input=("grid", 6, 3, 4, 2, 1),
+             args="-R 2 -g 2 -s 4 -n 80"),
+        dict(id="grid_full_g2_M", kind="grid", input=("gridsym", 6, 6, 4, 2, 2, 3),
+             args="-M -R 2 -g 2 -s 6 -n 80"),
+        dict(id="grid_full_g1_M", kind="grid", input=("grid", 8, 8, 4, 3, 5),
+             args="-M -R 3 -s 2 -n 100"),
+        dict(id="grid_lat_8x8_F", kind="grid", input=None,
+             args="-W 8 -H 8 -D 4 -R 3 -s 4 -n 150 -F 0.5"),
+        dict(id="grid_lat_8x8_Y", kind="grid", input=None, args="-W 8 -H 8 -D 4 -R 3 -s 4 -n 150 -Y"),
+        dict(id="grid_lat_10x10_C", kind="grid", input=None,
+             args="-W 10 -H 10 -D 4 -R 2 -s 7 -n 200 -C 10 -w 20 -c 1"),
+    ]
+
+
+def write_program_input(case: dict, path: str) -> None:
+    """Writes a program case's start file (the reference's file formats)."""
+    spec = case["input"]
+    if spec[0] == "regular":
+        _, n, d, seed = spec
+        G.write_edges(path, G.random_regular(n, d, seed))
+    elif spec[0] == "grid":
+        _, w, h, d, r, seed = spec
+        G.write_grid_edges(path, G.random_grid(w, h, d, r, seed), h)
+    else:
+        _, w, h, d, r, g, seed = spec
+        G.write_grid_edges(path, G.random_grid_symmetric(w, h, d, r, g, seed), h)
+
+
+def program_result_lines(text: str):
+    """The "Diam. k = ..." line and the step count of a program's output."""
+    diam = [ln for ln in text.splitlines() if ln.startswith("Diam. k = ")]
+    steps = [ln for ln in text.splitlines() if ln.startswith("Steps: ")]
+    n = int(steps[-1].split()[1].rstrip(",")) if steps else -1
+    return (diam[-1] if diam else ""), n

@@ -4,8 +4,10 @@ Runs in the development container only: it calls the reference's
 unmodified evaluation() (general/evaluation.c:375, grid/evaluation.c:192)
 compiled by oracle/Makefile into oracle/_ref/.  The CSV holds, per case,
 the reference's return value, diameter, ASPL (as a float hex string) and
-S = round(ASPL * n * (n-1)); and golden_sa.json, the outcome of the
-reference's sa() (general/sa.c:413-520) from fixed seeds.
+S = round(ASPL * n * (n-1)); golden_sa.json, the outcome of the
+reference's sa() (general/sa.c:413-520) from fixed seeds; and
+golden_programs.json, the output files of the reference's whole programs
+(main.c included) run on start files this build writes.
 
     python tests/golden/make_golden.py
 """
@@ -27,9 +29,10 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
 
 import oracle  # noqa: E402
-from cases import (GOLDEN_CSV, GOLDEN_GRID_SA, GOLDEN_NONREGULAR, GOLDEN_SA,  # noqa: E402
-                   build, build_nonregular, general_cases, grid_cases, grid_sa_cases,
-                   nonregular_cases, sa_cases)
+from cases import (GOLDEN_CSV, GOLDEN_GRID_SA, GOLDEN_NONREGULAR, GOLDEN_PROGRAMS,  # noqa: E402
+                   GOLDEN_SA, build, build_nonregular, general_cases, grid_cases, grid_sa_cases,
+                   nonregular_cases, program_cases, program_result_lines, sa_cases,
+                   write_program_input)
 from graphgolf_amd import graphs as G  # noqa: E402
 
 
@@ -72,6 +75,38 @@ def nonregular() -> None:
         w.writerows(rows)
 
 
+def programs() -> None:
+    """Output files of the reference's whole programs (general/main.c:317-497,
+    grid/main.c:586-777), built by oracle/Makefile into oracle/_ref/."""
+    import subprocess
+    import tempfile
+    rows = []
+    for case in program_cases():
+        with tempfile.TemporaryDirectory() as tmp:
+            inp, out = os.path.join(tmp, "in.txt"), os.path.join(tmp, "out.txt")
+            args = []
+            in_sha = ""
+            if case["input"] is not None:
+                write_program_input(case, inp)
+                with open(inp, "rb") as f:
+                    in_sha = hashlib.sha256(f.read()).hexdigest()
+                args = ["-f", inp]
+            exe = os.path.join(ROOT, "oracle", "_ref", case["kind"] + "_program")
+            res = subprocess.run([exe] + args + ["-o", out] + case["args"].split(), cwd=tmp,
+                                 capture_output=True, text=True, timeout=600,
+                                 env=dict(os.environ, OMP_NUM_THREADS="1"))
+            if res.returncode != 0:
+                raise SystemExit(f"{case['id']}: {res.stdout[-500:]} {res.stderr[-500:]}")
+            with open(out, "rb") as f:
+                out_sha = hashlib.sha256(f.read()).hexdigest()
+            diam_line, steps = program_result_lines(res.stdout)
+            rows.append(dict(case, input_sha256=in_sha, output_sha256=out_sha,
+                             result=diam_line, steps=steps))
+            print(rows[-1], flush=True)
+    with open(GOLDEN_PROGRAMS, "w") as f:
+        json.dump(rows, f, indent=1)
+
+
 def main() -> None:
     if not (oracle.ref_available("general") and oracle.ref_available("grid")):
         raise SystemExit("oracle/_ref is not built (make -C oracle ref)")
@@ -80,6 +115,9 @@ def main() -> None:
         return
     if "--nonregular" in sys.argv:
         nonregular()
+        return
+    if "--programs" in sys.argv:
+        programs()
         return
     rows = []
     for case in general_cases():
@@ -113,6 +151,7 @@ def main() -> None:
         json.dump(sa_rows, f, indent=1)
     grid_sa()
     nonregular()
+    programs()
 
 
 if __name__ == "__main__":
