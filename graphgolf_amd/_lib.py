@@ -46,7 +46,8 @@ class GGSaParams(ctypes.Structure):
     _fields_ = [("ncalcs", ctypes.c_int64), ("max_temp", ctypes.c_double),
                 ("min_temp", ctypes.c_double), ("cooling_cycle", ctypes.c_int32),
                 ("hill_climbing", ctypes.c_int32), ("seed", ctypes.c_uint32),
-                ("log_capacity", ctypes.c_int32)]
+                ("log_capacity", ctypes.c_int32), ("incremental", ctypes.c_int32),
+                ("pad", ctypes.c_int32)]
 
 
 class GGSaStats(ctypes.Structure):
@@ -134,6 +135,14 @@ def lib():
                                 ctypes.POINTER(GGSaParams), EVAL_FN, ctypes.c_void_p,
                                 ctypes.POINTER(GGSaStats), i32p, u64p], ctypes.c_int),
         "gg_grid_lattice": ([ctypes.c_int] * 4 + [ctypes.c_uint32, i32p], ctypes.c_int),
+        "gg_incr_create": ([ctypes.c_int, ctypes.c_int, i32p], ctypes.c_void_p),
+        "gg_incr_state": ([ctypes.c_void_p, ctypes.POINTER(GGResult)], ctypes.c_int),
+        "gg_incr_eval": ([ctypes.c_void_p, i32p, i32p, ctypes.POINTER(GGResult), i32p],
+                         ctypes.c_int),
+        "gg_incr_accept": ([ctypes.c_void_p], ctypes.c_int),
+        "gg_incr_reject": ([ctypes.c_void_p], ctypes.c_int),
+        "gg_incr_adjacency": ([ctypes.c_void_p, i32p], ctypes.c_int),
+        "gg_incr_destroy": ([ctypes.c_void_p], None),
         "gg_grid_program": ([ctypes.POINTER(GGGridShape), ctypes.c_int, i32p, ctypes.c_int, i32p,
                              ctypes.POINTER(GGSaParams), ctypes.c_double, EVAL_GRID_FN,
                              ctypes.c_void_p, ctypes.POINTER(GGSaStats), i32p, u64p],

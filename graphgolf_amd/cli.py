@@ -125,10 +125,13 @@ def _results(r, low_diam, low_aspl, elapsed, sep, out):
           f"Other: {elapsed - r.seconds_eval:f} sec.)", file=out)
 
 
-def run_general(argv, evaluator=None, out=sys.stdout):
+def run_general(argv, evaluator=None, out=sys.stdout, incremental=None):
     """The general program (general/main.c:317-497), groups = 1.  Returns the
     SaResult (edges = the best graph).  `evaluator` (adj -> (connected,
-    diameter, S)) replaces the device evaluator when given."""
+    diameter, S)) replaces the device evaluator when given.  `incremental`
+    (None: from n >= 4096 with degree <= 16 on the device evaluator) scores
+    proposals with the incremental scorer; its results are the same exact
+    integers, so the run and the output file are the same."""
     from . import graphs as G
     from . import sa as SA
 
@@ -181,8 +184,11 @@ def run_general(argv, evaluator=None, out=sys.stdout):
         print(f"Output filename: {outfile}", file=out)
     print("---", file=out)
     t0 = time.perf_counter()
+    if incremental is None:
+        incremental = evaluator is None and nodes >= 4096 and degree <= 16
     r = SA.general_program(edges, nodes, degree, seed, ncalcs, halfway=halfway, max_temp=max_t,
-                           min_temp=min_t, hill_climbing=hill, evaluator=evaluator)
+                           min_temp=min_t, hill_climbing=hill, evaluator=evaluator,
+                           incremental=incremental)
     if r.already_optimal:
         print("The input file has already optimum solution.", file=out)
         return r

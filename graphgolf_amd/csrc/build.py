@@ -32,7 +32,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-I", INC, 
 HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-I", INC, "-I", HERE, "-Wall",
               "-march=x86-64-v3"]
 
-HIP_SOURCES = ["gg_eval.hip", "gg_mlp.hip", "gg_bsa.hip"]
+HIP_SOURCES = ["gg_eval.hip", "gg_mlp.hip", "gg_bsa.hip", "gg_incr.hip"]
 HOST_SOURCES = ["gg_host.cpp"]
 
 

@@ -9,6 +9,10 @@ struct gg_mlp;
 namespace gg {
 // The evaluator's stream (adjacency uploads, plans, histograms run on it).
 hipStream_t engine_stream();
+// Launch shape of one tile-BFS launch of `bits`-wide words over an n-vertex
+// graph: threads per block, vertices per thread, reach-array capacity, queue
+// capacity and dynamic LDS bytes (the evaluator's own choice for that n).
+int tile_shape(int bits, int n, int *block, int *vpt, int *n_cap, int *qcap, size_t *lds);
 // Forward pass on m rows already in device memory (row-major, in_dim floats
 // each); copies the m predictions to y_host and waits for them.
 int mlp_forward_device(gg_mlp *M, const float *x_dev, int m, float *y_host);

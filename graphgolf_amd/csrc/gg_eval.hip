@@ -49,35 +49,7 @@
 #include "gg_host.h"
 #include "gg_internal.h"
 #include "gg_device.h"
-
-// ---------------------------------------------------------------------------
-// Device-side descriptors.
-
-struct GraphDev {
-  const uint16_t *adjR;  // [n][dp] row-major, padded with the row's own id
-  int32_t *flags;        // [0] = bad adjacency entries, [1] = asymmetric
-  int32_t n, dp, split, c;
-  int32_t push_ok, groups;
-  int32_t hist_maxd;     // >0: accumulate per-vertex distance histogram
-  int32_t trace_units;   // units whose per-level timings are recorded
-  int32_t *hist;         // [n][hist_maxd+1] (histogram mode only)
-  uint64_t *trace;       // [trace_units][kTraceLevels][2] or null
-};
-
-// Per-level trace record: (cycles since the unit started, mode word).
-constexpr int kTraceLevels = 64;
-
-struct UnitDesc {
-  int32_t graph;
-  int32_t first_row;
-  int32_t l;
-  int32_t centre;
-};
-
-struct UnitOut {
-  uint64_t acc_lo, acc_hi;
-  int32_t done_level, last_change, disconnected, error;
-};
+#include "gg_tile.h"
 
 struct GraphMeta {
   int32_t unit_begin, unit_count, n, groups;
@@ -88,94 +60,6 @@ struct GraphOut {
   int32_t connected, diameter, max_ecc, error;
 };
 
-// ---------------------------------------------------------------------------
-// Word helpers.
-
-template <typename W> struct WordBits { static constexpr int value = 8 * sizeof(W); };
-
-// Word 3 of a raw buffer resource on gfx9-family parts (32-bit data format).
-constexpr int kBufferWord3 = 0x00020000;
-
-// Pointer into the global address space.
-template <typename T> using gptr = __attribute__((address_space(1))) T *;
-
-template <typename W> __device__ __forceinline__ int wpopc(W x) {
-  if constexpr (sizeof(W) == 8) return __popcll((unsigned long long)x);
-  else return __popc((unsigned)x);
-}
-
-// LDS atomic OR of `f` into element x of R; returns the element's old value.
-template <typename W> __device__ __forceinline__ W lds_or(W *R, uint32_t x, W f) {
-  if constexpr (sizeof(W) == 8) {
-    return (W)atomicOr((unsigned long long *)(R + x), (unsigned long long)f);
-  } else if constexpr (sizeof(W) == 4) {
-    return (W)atomicOr((unsigned *)(R + x), (unsigned)f);
-  } else if constexpr (sizeof(W) == 2) {
-    unsigned *w = (unsigned *)R + (x >> 1);
-    unsigned sh = (x & 1u) * 16u;
-    return (W)(atomicOr(w, (unsigned)f << sh) >> sh);
-  } else {
-    unsigned *w = (unsigned *)R + (x >> 2);
-    unsigned sh = (x & 3u) * 8u;
-    return (W)(atomicOr(w, (unsigned)f << sh) >> sh);
-  }
-}
-
-// Register staging of VPT words per thread, packed into 32-bit registers.
-template <typename W, int VPT> struct StageRegs {
-  static constexpr int value = (VPT * (int)sizeof(W) + 3) / 4;
-};
-
-template <typename W> __device__ __forceinline__ void stage_put(uint32_t *st, int r, W val) {
-  if constexpr (sizeof(W) == 8) {
-    st[2 * r] = (uint32_t)val;
-    st[2 * r + 1] = (uint32_t)((uint64_t)val >> 32);
-  } else if constexpr (sizeof(W) == 4) {
-    st[r] = (uint32_t)val;
-  } else if constexpr (sizeof(W) == 2) {
-    if (r & 1) st[r >> 1] |= (uint32_t)val << 16;
-    else st[r >> 1] = (uint32_t)val;
-  } else {
-    if (r & 3) st[r >> 2] |= (uint32_t)val << ((r & 3) * 8);
-    else st[r >> 2] = (uint32_t)val;
-  }
-}
-
-template <typename W> __device__ __forceinline__ W stage_get(const uint32_t *st, int r) {
-  if constexpr (sizeof(W) == 8) return (W)(((uint64_t)st[2 * r + 1] << 32) | st[2 * r]);
-  else if constexpr (sizeof(W) == 4) return (W)st[r];
-  else if constexpr (sizeof(W) == 2) return (W)(st[r >> 1] >> ((r & 1) * 16));
-  else return (W)(st[r >> 2] >> ((r & 3) * 8));
-}
-
-// Cross-lane sums with DPP (VALU lane moves) instead of ds_bpermute, which
-// would queue behind the gathers in the LDS pipe.  DPP controls (GFX9):
-// quad_perm [1,0,3,2] = 0xB1, [2,3,0,1] = 0x4E; row_shr:k = 0x110 + k;
-// row_ror:k = 0x120 + k; row_bcast:15 = 0x142; row_bcast:31 = 0x143.
-#define GG_DPP(src, ctrl, rm, bm) __builtin_amdgcn_update_dpp(0u, (src), (ctrl), (rm), (bm), true)
-
-// Sum over the 64 lanes, returned in every lane (uniform).
-__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
-  v += GG_DPP(v, 0xB1, 0xf, 0xf);
-  v += GG_DPP(v, 0x4E, 0xf, 0xf);
-  v += GG_DPP(v, 0x124, 0xf, 0xf);
-  v += GG_DPP(v, 0x128, 0xf, 0xf);  // every lane: its 16-lane row's sum
-  v += GG_DPP(v, 0x142, 0xa, 0xf);  // rows 1, 3 += rows 0, 2
-  v += GG_DPP(v, 0x143, 0xc, 0xf);  // rows 2, 3 += rows 0-1
-  return __builtin_amdgcn_readlane(v, 63);
-}
-
-// Inclusive prefix sum over the lanes (lane i: sum of lanes 0..i).
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
-  uint32_t t = x + GG_DPP(x, 0x111, 0xf, 0xf);
-  t += GG_DPP(x, 0x112, 0xf, 0xf);
-  t += GG_DPP(x, 0x113, 0xf, 0xf);  // lanes i-3..i of the row
-  t += GG_DPP(t, 0x114, 0xf, 0xe);  // lanes 4-15 of a row: += 4 back
-  t += GG_DPP(t, 0x118, 0xf, 0xc);  // lanes 8-15: += 8 back (row prefix done)
-  t += GG_DPP(t, 0x142, 0xa, 0xf);  // rows 1, 3 += last lane of rows 0, 2
-  t += GG_DPP(t, 0x143, 0xc, 0xf);  // rows 2, 3 += lane 31
-  return t;
-}
 
 // ---------------------------------------------------------------------------
 // Adjacency preparation: row-major int32 [n][d] (+ optional relabel) ->
@@ -286,435 +170,6 @@ gg_patch_kernel(const int32_t *__restrict__ patches, int max_patch, int count,
   adjR0[slot * slot_stride + (size_t)ri * dp + j] = (uint16_t)rv;
 }
 
-// ---------------------------------------------------------------------------
-// The tile BFS.  One workgroup = one UnitDesc.  LDS layout (dynamic):
-//   R[n_cap] (W) | qb[2][qcap] (W) | qv[2][qcap] (u16) | red[2][16][4] | qcount[2]
-
-// 16 bytes of an adjacency row: 8 uint16 neighbour ids.
-typedef uint32_t Chunk __attribute__((ext_vector_type(4)));
-
-// Byte offset into a W array of the id held in the low (H = 0) or high
-// (H = 1) 16 bits of x: one SDWA shift per id instead of an extract and a
-// shift.
-template <typename W, int H> __device__ __forceinline__ uint32_t id_off(uint32_t x) {
-  constexpr int S = sizeof(W) == 1 ? 0 : sizeof(W) == 2 ? 1 : sizeof(W) == 4 ? 2 : 3;
-  uint32_t r;
-  if constexpr (H == 0)
-    asm("v_lshlrev_b32_sdwa %0, %2, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD "
-        "src1_sel:WORD_0"
-        : "=v"(r)
-        : "v"(x), "n"(S));
-  else
-    asm("v_lshlrev_b32_sdwa %0, %2, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD "
-        "src1_sel:WORD_1"
-        : "=v"(r)
-        : "v"(x), "n"(S));
-  return r;
-}
-
-// The reach array R starts at LDS address 0 (the kernel checks this on
-// entry), so an id's byte offset is its LDS address and the gathers need no
-// base add.
-template <typename W> __device__ __forceinline__ W word_at(uint32_t lds_addr) {
-  return *(const __attribute__((address_space(3))) W *)(size_t)lds_addr;
-}
-
-// OR of the reach words of 4 neighbour ids (two packed uint32 pairs).
-template <typename W> __device__ __forceinline__ W gather4(const W *, uint32_t a, uint32_t b) {
-  return (W)(word_at<W>(id_off<W, 0>(a)) | word_at<W>(id_off<W, 1>(a)) |
-             word_at<W>(id_off<W, 0>(b)) | word_at<W>(id_off<W, 1>(b)));
-}
-
-// ORs a chunk's 8 neighbours into val, in two halves, stopping once full.
-template <typename W> __device__ __forceinline__ W gather_chunk(const W *R, Chunk c, W val, W full) {
-  val = (W)(val | gather4<W>(R, c.x, c.y));
-  if (val != full) val = (W)(val | gather4<W>(R, c.z, c.w));
-  return val;
-}
-
-template <typename W, int BLOCK, int VPT, bool HIST>
-__global__ void __launch_bounds__(BLOCK)
-gg_tile_bfs_kernel(const GraphDev *__restrict__ graphs, const UnitDesc *__restrict__ units,
-                   UnitOut *__restrict__ outs, int unit_base, int n_cap, int qcap) {
-  constexpr int BITS = WordBits<W>::value;
-  constexpr int NWAVES = BLOCK / 64;
-  extern __shared__ __align__(16) unsigned char smem[];
-
-  const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
-  const int uidx = unit_base + blockIdx.x;
-  const UnitDesc U = units[uidx];
-  const GraphDev G = graphs[U.graph];
-  const int n = G.n, dp = G.dp, split = G.split;
-  const int nch = dp >> 3;  // 16-byte chunks per adjacency row
-  const int c = n - split;
-  const bool has_centres = c > 0;
-  // The descriptor's pointer is generic; say it is global so the gathers
-  // compile to global_load (not flat_load).
-  const gptr<const Chunk> rows = (gptr<const Chunk>)G.adjR;
-
-  size_t off = 0;
-  W *R = (W *)(smem + off);
-  off += ((size_t)n_cap * sizeof(W) + 15) & ~(size_t)15;
-  W *qb0 = (W *)(smem + off);
-  W *qb1 = qb0 + qcap;
-  off += ((size_t)2 * qcap * sizeof(W) + 15) & ~(size_t)15;
-  uint16_t *qv0 = (uint16_t *)(smem + off);
-  uint16_t *qv1 = qv0 + qcap;
-  off += ((size_t)2 * qcap * sizeof(uint16_t) + 15) & ~(size_t)15;
-  uint32_t *red = (uint32_t *)(smem + off);  // [2][NWAVES][4]
-  off += 2 * 16 * 4 * sizeof(uint32_t);
-  uint32_t *qcount = (uint32_t *)(smem + off);  // [2]
-
-  const int32_t bad = G.flags[0];
-  const bool lds_base_ok =
-      (uint32_t)(size_t)(const __attribute__((address_space(3))) unsigned char *)smem == 0u;
-  if (!lds_base_ok) {  // uniform: the gathers address R from LDS address 0
-    if (tid == 0) {
-      UnitOut o = {0, 0, n + 1, 0, 1, 2};
-      outs[uidx] = o;
-    }
-    return;
-  }
-  if (bad != 0) {  // uniform: adjacency had out-of-range entries
-    if (tid == 0) {
-      UnitOut o = {0, 0, n + 1, 0, 1, 1};
-      outs[uidx] = o;
-    }
-    return;
-  }
-  const bool push_ok = G.push_ok && G.flags[1] == 0;
-
-  const int l = U.l, first = U.first_row;
-  const bool centre = U.centre != 0;
-  const W full = (l >= BITS) ? (W)~(W)0 : (W)(((W)1 << l) - (W)1);
-
-  for (int v = tid; v < n; v += BLOCK) R[v] = (W)0;
-  if (tid == 0) {
-    qcount[0] = (uint32_t)l;
-    qcount[1] = 0;
-  }
-  __syncthreads();
-  if (tid < l) {
-    W b = (W)((W)1 << tid);
-    R[first + tid] = b;
-    qb0[tid] = b;
-    qv0[tid] = (uint16_t)(first + tid);
-    if constexpr (HIST) atomicAdd(&G.hist[(size_t)(first + tid) * (G.hist_maxd + 1)], 1);
-  }
-  __syncthreads();
-
-  const uint32_t lo_full = (uint32_t)split * (uint32_t)l;
-  const uint32_t hi_full = (uint32_t)c * (uint32_t)l;
-  int src_lo = split - first;
-  src_lo = src_lo < 0 ? 0 : (src_lo > l ? l : src_lo);
-  uint32_t tot_lo = (uint32_t)src_lo, tot_hi = (uint32_t)(l - src_lo);
-
-  uint64_t acc_lo = 0, acc_hi = 0;
-  int level = 0, done_level = n + 1, last_change = 0, disconnected = 0;
-  const bool tracing = G.trace != nullptr && uidx < G.trace_units;
-  const uint64_t t_start = tracing ? clock64() : 0;
-  bool qvalid = push_ok && l <= qcap;
-  uint32_t qn = (uint32_t)l;
-  int cur = 0;
-
-  while (true) {
-    ++level;
-    const bool do_push = qvalid && (qn * 4u <= (uint32_t)n);
-    uint32_t p_lo = 0, p_hi = 0, p_chg = 0;
-    // Staged pull results, packed into 32-bit registers (fully unrolled, so
-    // every index below is a compile-time constant).
-    uint32_t st[StageRegs<W, VPT>::value];
-    if (tid == 0) qcount[cur] = 0;  // this level reads qn (register); slot is next-next queue
-
-    // A push level records the next frontier only when it can plausibly fit
-    // the queue; otherwise the next level is a pull level anyway.
-    const bool build = do_push && qn * (uint32_t)dp <= 4u * (uint32_t)qcap;
-    if (do_push) {
-      // Top-down: push each frontier entry's new bits to its neighbours.
-      const W *qb_c = cur ? qb1 : qb0;
-      const uint16_t *qv_c = cur ? qv1 : qv0;
-      W *qb_n = cur ? qb0 : qb1;
-      uint16_t *qv_n = cur ? qv0 : qv1;
-      uint32_t *qc_n = &qcount[cur ^ 1];
-      // Wave-uniform trip count (the queue scan below needs every lane).
-      for (uint32_t e0 = (uint32_t)(tid - lane); e0 < qn; e0 += BLOCK) {
-        const uint32_t e = e0 + (uint32_t)lane;
-        const bool valid = e < qn;
-        const uint32_t u = valid ? (uint32_t)qv_c[e] : 0u;
-        const W f = valid ? qb_c[e] : (W)0;
-        for (int ch = 0; ch < nch; ++ch) {
-          Chunk cv = {0u, 0u, 0u, 0u};
-          if (valid) cv = rows[(size_t)u * nch + ch];
-          const uint32_t ids[8] = {cv.x & 0xffffu, cv.x >> 16, cv.y & 0xffffu, cv.y >> 16,
-                                   cv.z & 0xffffu, cv.z >> 16, cv.w & 0xffffu, cv.w >> 16};
-          // Eight independent LDS atomics first, then the bookkeeping.
-          W nbs[8];
-#pragma unroll
-          for (int k = 0; k < 8; ++k) nbs[k] = (W)0;
-          if (valid) {
-#pragma unroll
-            for (int k = 0; k < 8; ++k) nbs[k] = (W)(f & (W)~lds_or<W>(R, ids[k], f));
-          }
-          uint32_t mine = 0;
-#pragma unroll
-          for (int k = 0; k < 8; ++k) {
-            if (nbs[k]) {
-              const uint32_t pc = (uint32_t)wpopc<W>(nbs[k]);
-              if ((int)ids[k] < split) p_lo += pc;
-              else p_hi += pc;
-              ++mine;
-              if constexpr (HIST)
-                atomicAdd(&G.hist[(size_t)ids[k] * (G.hist_maxd + 1) + min(level, G.hist_maxd)],
-                          (int)pc);
-            }
-          }
-          if (build) {
-            // One queue reservation per wave: exclusive scan of the lanes'
-            // counts, the last active lane reserves the total.
-            const uint32_t incl = wave_incl_scan(mine);
-            const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
-            uint32_t base = 0;
-            if (lane == 63 && total) base = atomicAdd(qc_n, total);
-            base = __builtin_amdgcn_readlane(base, 63);
-            uint32_t slot = base + incl - mine;
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-              if (nbs[k]) {
-                if (slot < (uint32_t)qcap) {
-                  qb_n[slot] = nbs[k];
-                  qv_n[slot] = (uint16_t)ids[k];
-                }
-                ++slot;
-              }
-            }
-          }
-        }
-      }
-    } else {
-      // Bottom-up: every vertex ORs its neighbours' words, stopping once its
-      // own word is full (tested every 4 ids on dense levels).  `t` is
-      // laundered so that per-vertex addresses are recomputed inside this
-      // level instead of being hoisted out of the level loop (which would pin
-      // VPT of them in registers).
-      int t = tid;
-      asm volatile("" : "+v"(t));
-      const uint32_t row_bytes = (uint32_t)nch * 16u;
-      const gptr<const char> row_base = (gptr<const char>)G.adjR;
-      const __amdgpu_buffer_rsrc_t row_rsrc = __builtin_amdgcn_make_buffer_rsrc(
-          (void *)G.adjR, (short)0, (int)((uint32_t)n * row_bytes), kBufferWord3);
-      // Below a quarter of the source-vertex pairs reached, a vertex rarely
-      // fills up within its first 16 neighbours: rows of two chunks then
-      // gather all 16 without the per-4 early-exit tests (more loads in
-      // flight per wave, straight-line code).
-      const bool sparse2 =
-          nch == 2 && 4ull * (uint64_t)(tot_lo + tot_hi) < (uint64_t)(lo_full + hi_full);
-      // Fewer than 1/32 of the source-vertex pairs still unreached.
-      const bool very_dense =
-          32ull * ((uint64_t)(lo_full + hi_full) - (uint64_t)(tot_lo + tot_hi)) <
-          (uint64_t)(lo_full + hi_full);
-      W chg_acc = (W)0;
-      uint32_t p_all = 0;
-      // Buffer loads of a row's first two chunks (the second one is always
-      // loaded: for a one-chunk row it is the next row's first chunk, or zeros
-      // past the table, and is not used).  A full vertex (or one past n) gets
-      // an offset outside the buffer, which the hardware answers with zeros
-      // without touching memory, so the loads need no branch and the compiler
-      // waits for exactly the row being used.
-      auto fetch_rows = [&](int v, W o, Chunk &x, Chunk &y) {
-        const uint32_t off = o != full ? (uint32_t)v * row_bytes : 0x80000000u;
-        x = __builtin_amdgcn_raw_buffer_load_b128(row_rsrc, off, 0, 0);
-        y = __builtin_amdgcn_raw_buffer_load_b128(row_rsrc, off + 16u, 0, 0);
-      };
-      // Software pipeline over this thread's vertices: the word of vertex
-      // r + 3 is read from LDS while vertex r gathers, so that the row load
-      // of vertex r + 2 (issued before vertex r gathers; out of range, hence
-      // free of memory traffic, for a full vertex) does not wait behind the
-      // gathers in the LDS queue.  Words rotate through four slots and rows
-      // through three; the loop is fully unrolled, so every slot choice is
-      // resolved at compile time.
-      if (very_dense) {
-        // Almost every vertex is already full: no prefetching; the few that
-        // are not load their row when they get to it.
-        const gptr<const Chunk> rows_g = (gptr<const Chunk>)G.adjR;
-#pragma clang loop unroll(full)
-        for (int r = 0; r < VPT; ++r) {
-          const int v = r * BLOCK + t;
-          const W old = (v < n) ? R[v] : full;
-          W val = old;
-          if (val != full) {
-            for (int ch = 0; ch < nch && val != full; ++ch)
-              val = gather_chunk<W>(R, rows_g[(uint32_t)v * (uint32_t)nch + (uint32_t)ch], val, full);
-          }
-          if (v < n) {
-            const uint32_t pc = (uint32_t)wpopc<W>(val);
-            p_all += pc;
-            if (has_centres && v >= split) p_hi += pc;  // uniform test first
-            chg_acc = (W)(chg_acc | (val ^ old));
-            if constexpr (HIST) {
-              if (val != old)
-                atomicAdd(&G.hist[(size_t)v * (G.hist_maxd + 1) + min(level, G.hist_maxd)],
-                          wpopc<W>((W)(val & (W)~old)));
-            }
-          }
-          stage_put<W>(st, r, val);
-          __builtin_amdgcn_sched_barrier(0);
-        }
-      } else {
-        W o0 = full, o1 = full, o2 = full, o3 = full;
-        Chunk a0, b0, a1, b1, a2, b2;
-        auto word = [&](int v) { return (v < n) ? R[v] : full; };
-        o0 = word(t);
-        if (VPT > 1) o1 = word(BLOCK + t);
-        if (VPT > 2) o2 = word(2 * BLOCK + t);
-        fetch_rows(t, o0, a0, b0);
-        if (VPT > 1) fetch_rows(BLOCK + t, o1, a1, b1);
-  #pragma clang loop unroll(full)
-        for (int r = 0; r < VPT; ++r) {
-          const int v = r * BLOCK + t;
-          if (r + 3 < VPT) {
-            const W o = word(v + 3 * BLOCK);
-            const int q = (r + 3) % 4;
-            if (q == 0) o0 = o;
-            else if (q == 1) o1 = o;
-            else if (q == 2) o2 = o;
-            else o3 = o;
-          }
-          if (r + 2 < VPT) {
-            const int q = (r + 2) % 4;
-            const W o = q == 0 ? o0 : q == 1 ? o1 : q == 2 ? o2 : o3;
-            Chunk x, y;
-            fetch_rows(v + 2 * BLOCK, o, x, y);
-            const int s2 = (r + 2) % 3;
-            if (s2 == 0) {
-              a0 = x;
-              b0 = y;
-            } else if (s2 == 1) {
-              a1 = x;
-              b1 = y;
-            } else {
-              a2 = x;
-              b2 = y;
-            }
-          }
-          const int q = r % 4, s = r % 3;
-          const W old = q == 0 ? o0 : q == 1 ? o1 : q == 2 ? o2 : o3;
-          const Chunk ca = s == 0 ? a0 : s == 1 ? a1 : a2;
-          const Chunk cb = s == 0 ? b0 : s == 1 ? b1 : b2;
-          W val = old;
-          if (v < n) {
-            if (val != full) {
-              if (sparse2) {  // uniform: two-chunk rows on a sparse level, 16 ids at once
-                val = (W)(val | gather4<W>(R, ca.x, ca.y) | gather4<W>(R, ca.z, ca.w) |
-                          gather4<W>(R, cb.x, cb.y) | gather4<W>(R, cb.z, cb.w));
-              } else {
-                val = gather_chunk<W>(R, ca, val, full);
-                if (nch > 1 && val != full) val = gather_chunk<W>(R, cb, val, full);
-                if (nch > 2) {
-                  const gptr<const Chunk> rp = (gptr<const Chunk>)(row_base + (uint32_t)v * row_bytes);
-                  for (int ch = 2; ch < nch && val != full; ++ch)
-                    val = gather_chunk<W>(R, rp[ch], val, full);
-                }
-              }
-            }
-            const uint32_t pc = (uint32_t)wpopc<W>(val);
-            p_all += pc;
-            if (has_centres && v >= split) p_hi += pc;  // uniform test first
-            chg_acc = (W)(chg_acc | (val ^ old));
-            if constexpr (HIST) {
-              if (val != old)
-                atomicAdd(&G.hist[(size_t)v * (G.hist_maxd + 1) + min(level, G.hist_maxd)],
-                          wpopc<W>((W)(val & (W)~old)));
-            }
-          }
-          stage_put<W>(st, r, val);
-          // Keep the scheduler from interleaving vertices (register pressure);
-          // the prefetches issued above stay in flight across this point.
-          __builtin_amdgcn_sched_barrier(0);
-        }
-      }
-      p_lo = p_all - p_hi;
-      p_chg = chg_acc != (W)0 ? 1u : 0u;
-    }
-
-    // Block reduction of (lo, hi, changed): per wave with DPP, across waves
-    // through LDS (one 16-byte record per wave, double-buffered by level).
-    p_lo = wave_sum(p_lo);
-    p_hi = wave_sum(p_hi);
-    p_chg = __builtin_amdgcn_ballot_w64(p_chg != 0u) != 0ull ? 1u : 0u;
-    uint32_t *rb = red + (level & 1) * 16 * 4;
-    if (lane == 0) {
-      rb[wave * 4 + 0] = p_lo;
-      rb[wave * 4 + 1] = p_hi;
-      rb[wave * 4 + 2] = p_chg;
-    }
-    __syncthreads();  // B1: all reads of R for this level are done
-    if (!do_push) {
-      int t = tid;
-      asm volatile("" : "+v"(t));
-#pragma unroll
-      for (int r = 0; r < VPT; ++r) {
-        const int v = r * BLOCK + t;
-        if (v < n) R[v] = stage_get<W>(st, r);
-      }
-    }
-    // Lane k < NWAVES reads wave k's record; DPP sums finish the reduction.
-    uint32_t r_lo = 0, r_hi = 0, r_chg = 0;
-    if (lane < NWAVES) {
-      const Chunk rec = *(const Chunk *)(rb + lane * 4);
-      r_lo = rec.x;
-      r_hi = rec.y;
-      r_chg = rec.z;
-    }
-    const uint32_t L_lo = wave_sum(r_lo), L_hi = wave_sum(r_hi), L_chg = wave_sum(r_chg);
-    const uint32_t qn_next = qcount[cur ^ 1];
-    __syncthreads();  // B2: R write-back visible, counters read
-
-    bool chg;
-    if (do_push) {
-      tot_lo += L_lo;
-      tot_hi += L_hi;
-      chg = (L_lo + L_hi) != 0;
-      qvalid = build && qn_next <= (uint32_t)qcap;
-      qn = qn_next;
-      cur ^= 1;
-    } else {
-      tot_lo = L_lo;
-      tot_hi = L_hi;
-      chg = L_chg != 0;
-      qvalid = false;
-    }
-    if (chg) last_change = level;
-    if (tracing && tid == 0 && level < kTraceLevels) {
-      uint64_t *tr = G.trace + ((size_t)uidx * kTraceLevels + level) * 2;
-      tr[0] = clock64() - t_start;
-      tr[1] = (uint64_t)(do_push ? 1 : 0) | ((uint64_t)(do_push ? qn_next : L_lo + L_hi) << 1);
-    }
-    const bool done = centre ? (tot_hi == hi_full) : (tot_lo + tot_hi == lo_full + hi_full);
-    if (done) {
-      done_level = level;
-      break;
-    }
-    if (!chg || level >= n) {
-      disconnected = 1;
-      done_level = n + 1;
-      break;
-    }
-    if (!centre) acc_lo += (uint64_t)(lo_full - tot_lo);
-    acc_hi += (uint64_t)(hi_full - tot_hi);
-  }
-
-  if (tid == 0) {
-    UnitOut o;
-    o.acc_lo = acc_lo;
-    o.acc_hi = acc_hi;
-    o.done_level = done_level;
-    o.last_change = last_change;
-    o.disconnected = disconnected;
-    o.error = 0;
-    outs[uidx] = o;
-  }
-}
 
 // Per-graph combination of the unit partial sums
 // (general/evaluation.c:199-200, :241, :247-249 as integers).  One
@@ -863,12 +318,12 @@ struct KernelKey {
 
 typedef void (*TileKernel)(const GraphDev *, const UnitDesc *, UnitOut *, int, int, int);
 
-template <typename W, int B, int V, bool H>
+template <typename W, int B, int V, int H>
 static TileKernel kptr() {
   return (TileKernel)&gg_tile_bfs_kernel<W, B, V, H>;
 }
 
-template <typename W, bool H>
+template <typename W, int H>
 static TileKernel pick_vpt(int block, int vpt) {
   if (block == 256) {
     switch (vpt) {
@@ -891,13 +346,15 @@ static TileKernel pick_vpt(int block, int vpt) {
   return nullptr;
 }
 
+// k.hist: kTileNormal or kTileHist (the source-list modes are instantiated
+// by the incremental scorer).
 static TileKernel kernel_for(const KernelKey &k) {
   const bool h = k.hist != 0;
   switch (k.bits) {
-    case 8: return h ? pick_vpt<uint8_t, true>(k.block, k.vpt) : pick_vpt<uint8_t, false>(k.block, k.vpt);
-    case 16: return h ? pick_vpt<uint16_t, true>(k.block, k.vpt) : pick_vpt<uint16_t, false>(k.block, k.vpt);
-    case 32: return h ? pick_vpt<uint32_t, true>(k.block, k.vpt) : pick_vpt<uint32_t, false>(k.block, k.vpt);
-    case 64: return h ? pick_vpt<uint64_t, true>(k.block, k.vpt) : pick_vpt<uint64_t, false>(k.block, k.vpt);
+    case 8: return h ? pick_vpt<uint8_t, kTileHist>(k.block, k.vpt) : pick_vpt<uint8_t, kTileNormal>(k.block, k.vpt);
+    case 16: return h ? pick_vpt<uint16_t, kTileHist>(k.block, k.vpt) : pick_vpt<uint16_t, kTileNormal>(k.block, k.vpt);
+    case 32: return h ? pick_vpt<uint32_t, kTileHist>(k.block, k.vpt) : pick_vpt<uint32_t, kTileNormal>(k.block, k.vpt);
+    case 64: return h ? pick_vpt<uint64_t, kTileHist>(k.block, k.vpt) : pick_vpt<uint64_t, kTileNormal>(k.block, k.vpt);
   }
   return nullptr;
 }
@@ -918,8 +375,9 @@ static size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
 
 static size_t lds_bytes_for(int bits, int n_cap, int qcap) {
   size_t wb = bits / 8;
+  // red[2][16][4] | qcount[2] (+2 pad) | sacc[2][64] (kTileSrc reached counts)
   return align16((size_t)n_cap * wb) + align16((size_t)2 * qcap * wb) +
-         align16((size_t)2 * qcap * 2) + 2 * 16 * 4 * 4 + 16;
+         align16((size_t)2 * qcap * 2) + 2 * 16 * 4 * 4 + 16 + 2 * 64 * 4;
 }
 
 // The largest queue that fits the LDS budget, capped at what push levels
@@ -992,6 +450,18 @@ void finalize() {
   hipStreamDestroy(E.stream);
   E.stream = nullptr;
   E.ready = false;
+}
+
+int tile_shape(int bits, int n, int *block, int *vpt, int *n_cap, int *qcap, size_t *lds) {
+  block_vpt_for(n, block, vpt);
+  *n_cap = (n + 7) & ~7;
+  const size_t rbytes = (size_t)*n_cap * (bits / 8);
+  const size_t budget =
+      rbytes * 2 + 4096 <= (size_t)kLdsBytes / 2 ? rbytes * 2 + 4096 : (size_t)kLdsBytes;
+  *qcap = qcap_for(bits, *n_cap, budget);
+  if (*qcap < 64) return set_error(GG_ERR_TOO_LARGE, "LDS budget too small for n = %d", n);
+  *lds = lds_bytes_for(bits, *n_cap, *qcap);
+  return GG_OK;
 }
 
 int max_bits_for(int n) {

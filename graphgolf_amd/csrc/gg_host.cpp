@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "gg_eval.h"
+#include "gg_incr.h"
 #include "gg_host.h"
 #include "gg_internal.h"
 #include "gg_sa_common.h"
@@ -470,6 +471,8 @@ int sa_general_run(int nodes, int degree, int32_t *edges, const gg_sa_params *P,
                    bool seed_rng) {
   if (!edges || !P || nodes < 3 || degree < 1 || ((int64_t)nodes * degree) % 2)
     return set_error(GG_ERR_INVALID, "bad SA arguments");
+  if (P->incremental && eval)
+    return set_error(GG_ERR_INVALID, "incremental scoring runs on the device evaluator only");
   if (!eval) eval = device_eval;
   const double t_start = now_s();
   double t_eval = 0;
@@ -477,11 +480,38 @@ int sa_general_run(int nodes, int degree, int32_t *edges, const gg_sa_params *P,
   const int lines = S.lines;
   std::vector<int32_t> best(edges, edges + 2 * (size_t)lines);
   int64_t logn = 0, evals = 0;
-  auto evaluate = [&](int *diam, double *aspl) -> int {
+  // Incremental scorer: the same exact (diameter, S) per proposal, from the
+  // sources the move can affect (include/gg_incr.h).
+  struct IncrGuard {
+    gg_incr *p = nullptr;
+    ~IncrGuard() { gg_incr_destroy(p); }
+  } inc;
+  if (P->incremental) {
+    inc.p = gg_incr_create(nodes, degree, S.adj.data());
+    if (!inc.p) return GG_ERR_INVALID;
+  }
+  // m: the proposal just applied to S (nullptr: the current graph).
+  auto evaluate = [&](int *diam, double *aspl, const Move *m) -> int {
     int32_t dm = 0, conn = 0;
     uint64_t sum = 0;
     double t0 = now_s();
-    int rc = eval(ctx, nodes, degree, S.adj.data(), &dm, &sum, &conn);
+    int rc;
+    if (inc.p) {
+      gg_result r{};
+      if (!m) {
+        rc = gg_incr_state(inc.p, &r);
+      } else {
+        const int32_t rem[4] = {m->old_edge[0], m->old_edge[1], m->old_edge[2], m->old_edge[3]};
+        const int32_t add[4] = {edges[2 * m->line[0]], edges[2 * m->line[0] + 1],
+                                edges[2 * m->line[1]], edges[2 * m->line[1] + 1]};
+        rc = gg_incr_eval(inc.p, rem, add, &r, nullptr);
+      }
+      dm = r.diameter;
+      sum = r.sum;
+      conn = r.connected;
+    } else {
+      rc = eval(ctx, nodes, degree, S.adj.data(), &dm, &sum, &conn);
+    }
     t_eval += now_s() - t0;
     if (rc) return rc;
     ++evals;
@@ -508,7 +538,7 @@ int sa_general_run(int nodes, int degree, int32_t *edges, const gg_sa_params *P,
   if (seed_rng) srandom(P->seed);
   int cur_diam, best_diam;
   double cur_aspl, best_aspl;
-  int rc = evaluate(&cur_diam, &cur_aspl);
+  int rc = evaluate(&cur_diam, &cur_aspl, nullptr);
   if (rc < 0) return rc;
   best_diam = cur_diam;
   best_aspl = cur_aspl;
@@ -521,10 +551,11 @@ int sa_general_run(int nodes, int degree, int32_t *edges, const gg_sa_params *P,
     while (true) {
       if (restore) S.restore(mv);
       if ((rc = S.propose(mv)) != GG_OK) return rc;
-      rc = evaluate(&nd, &na);
+      rc = evaluate(&nd, &na, &mv);
       if (rc < 0) return rc;
       if (rc == 1) break;
       restore = true;  // disconnected: undo and propose again
+      if (inc.p && (rc = gg_incr_reject(inc.p))) return rc;
     }
     bool acc;
     if (nd < cur_diam) acc = true;
@@ -535,6 +566,7 @@ int sa_general_run(int nodes, int degree, int32_t *edges, const gg_sa_params *P,
       const double diff = ((cur_aspl - na) * nodes * (nodes - 1)) / 1;  // groups == 1
       acc = std::exp(diff / temp) > ref_uniform();
     }
+    if (inc.p && (rc = acc ? gg_incr_accept(inc.p) : gg_incr_reject(inc.p))) return rc;
     if (!acc) {
       restore = true;
     } else {
@@ -601,6 +633,7 @@ extern "C" int gg_general_program(int nodes, int degree, int32_t *edges, int hal
     return set_error(GG_ERR_INVALID, "bad program arguments (nodes %d degree %d)", nodes, degree);
   int rc = check_simple_regular(nodes, degree, edges);
   if (rc) return rc;
+  const gg_eval_fn user_eval = eval;
   if (!eval) eval = device_eval;
   const int lines = nodes * degree / 2;
   auto connected = [&](const int32_t *adj, int *diam, double *aspl) -> int {
@@ -652,7 +685,7 @@ extern "C" int gg_general_program(int nodes, int degree, int32_t *edges, int hal
     for (int i = 0; i < 5; ++i)
       if ((rc = T.propose(mv))) return rc;
   }
-  return sa_general_run(nodes, degree, edges, P, eval, ctx, stats, log_diam, log_sum, false);
+  return sa_general_run(nodes, degree, edges, P, user_eval, ctx, stats, log_diam, log_sum, false);
 }
 
 

@@ -79,11 +79,15 @@ def _result(rc, e, st, log, ld, ls) -> SaResult:
 
 def sa_general(edges, nodes: int, degree: int, seed: int, ncalcs: int, max_temp: float = 100.0,
                min_temp: float = 0.217147, cooling_cycle: int = 1, hill_climbing: bool = False,
-               evaluator: Optional[Callable] = None, log: int = 0) -> SaResult:
+               evaluator: Optional[Callable] = None, log: int = 0,
+               incremental: bool = False) -> SaResult:
     """Plain SA on a general graph (groups = 1).  `evaluator(adj) ->
-    (connected, diameter, S)` replaces the device evaluator when given."""
+    (connected, diameter, S)` replaces the device evaluator when given;
+    `incremental` scores proposals with the incremental scorer (same exact
+    results, so the same trajectory)."""
     e = L.c32(edges).reshape(-1, 2).copy()
-    p = L.GGSaParams(ncalcs, max_temp, min_temp, cooling_cycle, int(hill_climbing), seed, log)
+    p = L.GGSaParams(ncalcs, max_temp, min_temp, cooling_cycle, int(hill_climbing), seed, log,
+                     int(incremental))
     st = L.GGSaStats()
     ld = np.zeros(max(log, 1), dtype=np.int32)
     ls = np.zeros(max(log, 1), dtype=np.uint64)
@@ -95,13 +99,15 @@ def sa_general(edges, nodes: int, degree: int, seed: int, ncalcs: int, max_temp:
 
 def general_program(edges, nodes: int, degree: int, seed: int, ncalcs: int, halfway: bool = False,
                     max_temp: float = 100.0, min_temp: float = 0.217147, hill_climbing: bool = False,
-                    evaluator: Optional[Callable] = None, log: int = 0) -> SaResult:
+                    evaluator: Optional[Callable] = None, log: int = 0,
+                    incremental: bool = False) -> SaResult:
     """The general program's run from its start graph (general/main.c:360-462):
     the pre-run moves unless `halfway` (-M), the start-graph check, then SA, all
     on one random() stream seeded with `seed`.  `already_optimal` is set when
     the start graph meets the Moore bound (nothing run)."""
     e = L.c32(edges).reshape(-1, 2).copy()
-    p = L.GGSaParams(ncalcs, max_temp, min_temp, 1, int(hill_climbing), seed, log)
+    p = L.GGSaParams(ncalcs, max_temp, min_temp, 1, int(hill_climbing), seed, log,
+                     int(incremental))
     st = L.GGSaStats()
     ld = np.zeros(max(log, 1), dtype=np.int32)
     ls = np.zeros(max(log, 1), dtype=np.uint64)

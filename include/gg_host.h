@@ -61,6 +61,10 @@ typedef struct gg_sa_params {
   int32_t hill_climbing; /* -H */
   uint32_t seed;         /* srandom(seed) */
   int32_t log_capacity;  /* entries available in log_* arrays (0 = no log) */
+  int32_t incremental;   /* general SA on the device evaluator: score each proposal
+                            with the incremental scorer (include/gg_incr.h; simple graphs,
+                            degree <= 16).  Same exact results, so the same trajectory. */
+  int32_t pad;
 } gg_sa_params;
 
 typedef struct gg_sa_stats {

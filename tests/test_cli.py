@@ -33,7 +33,7 @@ def oracle_grid_eval(adj, groups, based_nodes, rotate_hash):
     return r.ok, r.diameter, r.S
 
 
-def run_program_case(case, tmp_path, device: bool):
+def run_program_case(case, tmp_path, device: bool, incremental=None):
     """Runs one golden program case through the CLI; checks it against the
     reference program's recorded output."""
     row = GOLD[case["id"]]
@@ -48,7 +48,8 @@ def run_program_case(case, tmp_path, device: bool):
     args += ["-o", out] + case["args"].split()
     buf = io.StringIO()
     if case["kind"] == "general":
-        cli.run_general(args, evaluator=None if device else oracle_eval, out=buf)
+        cli.run_general(args, evaluator=None if device else oracle_eval, out=buf,
+                        incremental=incremental)
     else:
         cli.run_grid(args, evaluator=None if device else oracle_grid_eval, out=buf)
     with open(out, "rb") as f:

@@ -4,6 +4,7 @@ reproduce the reference's sa() trajectory."""
 import hashlib
 import json
 
+import numpy as np
 import pytest
 
 import oracle
@@ -29,6 +30,36 @@ def test_gpu_sa_matches_reference_trajectory(case):
     assert got.best_diameter == row["best_diameter"]
     assert got.best_aspl == float.fromhex(row["best_aspl_hex"])
     assert hashlib.sha256(got.edges.tobytes()).hexdigest() == row["edges_sha256"]
+
+
+@pytest.mark.parametrize("case", [c for c in sa_cases() if c["d"] <= 16], ids=lambda c: c["id"])
+def test_gpu_sa_incremental_matches_reference_trajectory(case):
+    """The same runs with every proposal scored by the incremental scorer."""
+    e0 = G.random_regular(case["n"], case["d"], case["graph_seed"])
+    got = SA.sa_general(e0, case["n"], case["d"], case["seed"], case["ncalcs"],
+                        cooling_cycle=case.get("cooling_cycle", 1),
+                        hill_climbing=case.get("hill", False), incremental=True)
+    row = GOLD[case["id"]]
+    assert got.steps == row["steps"]
+    assert got.best_diameter == row["best_diameter"]
+    assert got.best_aspl == float.fromhex(row["best_aspl_hex"])
+    assert hashlib.sha256(got.edges.tobytes()).hexdigest() == row["edges_sha256"]
+
+
+@pytest.mark.parametrize("n,d,steps", [(4096, 16, 300), (65536, 16, 60)])
+def test_gpu_sa_incremental_same_log_as_full(n, d, steps):
+    """Every evaluation of the run, (diameter, S), is the same with and
+    without the incremental scorer, at the C2 and C5 sizes."""
+    print(f"sa incremental vs full n={n}", flush=True)
+    e0 = G.random_regular(n, d, 11)
+    full = SA.sa_general(e0, n, d, 5, steps, max_temp=50.0, min_temp=1.0, log=4 * steps)
+    inc = SA.sa_general(e0, n, d, 5, steps, max_temp=50.0, min_temp=1.0, log=4 * steps,
+                        incremental=True)
+    assert full.evaluations == inc.evaluations
+    assert np.array_equal(full.log_diameter, inc.log_diameter)
+    assert np.array_equal(full.log_sum, inc.log_sum)
+    assert np.array_equal(full.edges, inc.edges)
+    assert full.accepts == inc.accepts and full.accepts > 0
 
 
 def test_gpu_sa_every_evaluation_checked_against_oracle():
