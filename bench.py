@@ -126,12 +126,20 @@ def sa_time_to_aspl(budget_s):
                           use_surrogate=False, exact=1, seed=1)
     target = float(plain.log_best_aspl[-1])
     t_plain = float(plain.log_time[int(np.argmax(plain.log_best_aspl <= target))])
+    # The same plain run (same seed, hence the same moves and decisions) with
+    # every proposal scored by the incremental scorer.
+    inc = SA.sa_batched(e0, n, d, max_steps=10**9, time_budget_s=budget_s, target_aspl=target,
+                        use_surrogate=False, exact=1, seed=1, incremental=True)
+    t_inc = float(inc.stats["seconds_to_target"])
     sur = SA.sa_batched(e0, n, d, max_steps=10**9, time_budget_s=budget_s, target_aspl=target,
                         proposals=4096, combine=16, warmup=200, explore_every=8, seed=1)
     t_sur = float(sur.stats["seconds_to_target"])
     return {"workload": "sa n=65536 d=16 (random regular start, seed 5)",
             "target_aspl": target, "plain_seconds": round(t_plain, 3),
             "plain_evaluations": int(plain.stats["evaluations"]),
+            "plain_incremental_seconds": round(t_inc, 3) if t_inc > 0 else None,
+            "plain_incremental_same_run": bool(inc.stats["evaluations"] <= plain.stats["evaluations"]
+                                               and inc.stats["best_aspl"] <= target),
             "surrogate_seconds": round(t_sur, 3) if t_sur > 0 else None,
             "surrogate_evaluations": int(sur.stats["evaluations"]),
             "surrogate_moves_accepted": int(sur.stats["moves_accepted"]),

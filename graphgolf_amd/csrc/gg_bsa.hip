@@ -21,6 +21,7 @@
 #include "gg_device.h"
 #include "gg_eval.h"
 #include "gg_host.h"
+#include "gg_incr.h"
 #include "gg_internal.h"
 #include "gg_mlp.h"
 #include "gg_sa_common.h"
@@ -478,6 +479,28 @@ extern "C" int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa
   std::vector<int32_t> best_edges(edges, edges + 2 * (size_t)g.lines);
   Rng rng(P->seed);
   hipStream_t es = gg::engine_stream();
+  // Incremental scorer for single-move candidates (plain runs only).
+  struct IncrGuard {
+    gg_incr *p = nullptr;
+    ~IncrGuard() { gg_incr_destroy(p); }
+  } incr;
+  bool incr_pending = false;
+  if (P->incremental) {
+    if (use_sur) return set_error(GG_ERR_INVALID, "incremental scoring is for runs without the surrogate");
+    incr.p = gg_incr_create(nodes, degree, g.adj.data());
+    if (!incr.p) return GG_ERR_INVALID;
+  }
+  // The move of a single-move candidate as (removed, added) edges.
+  auto move_edges = [](const Prop &p, int32_t rem[4], int32_t add[4]) {
+    rem[0] = p.a;
+    rem[1] = p.b;
+    rem[2] = p.c;
+    rem[3] = p.d;
+    add[0] = p.a;
+    add[1] = p.r ? p.c : p.d;
+    add[2] = p.r ? p.b : p.c;
+    add[3] = p.r ? p.d : p.b;
+  };
 
   gg_graph shape = {nodes, degree, nodes, 1, 0, GG_GRAPH_SYMMETRIC, g.adj.data(), nullptr};
   PlanHolder plan;
@@ -634,9 +657,21 @@ extern "C" int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa
           patches[((size_t)c * max_patch + 4 * k + j) * 2 + 1] = cand[c].sl[k].val[j];
         }
     const double te = now_s();
-    rc = gg_plan_set_patches(plan.p, patches.data());
-    if (!rc) rc = gg_plan_launch(plan.p);
-    if (rc) return rc;
+    if (incr.p) {
+      // One candidate stays pending for the accept decision; with several,
+      // each is scored and dropped, and the accepted one is scored again.
+      for (int c = 0; c < ncand; ++c) {
+        int32_t rem[4], add[4];
+        move_edges(props[cand[c].prop[0]], rem, add);
+        if ((rc = gg_incr_eval(incr.p, rem, add, &res[c], nullptr))) return rc;
+        if (ncand > 1 && (rc = gg_incr_reject(incr.p))) return rc;
+      }
+      incr_pending = ncand == 1;
+    } else {
+      rc = gg_plan_set_patches(plan.p, patches.data());
+      if (!rc) rc = gg_plan_launch(plan.p);
+      if (rc) return rc;
+    }
     // While the candidates are scored, train on the rows stored so far (the
     // network's own stream; this step's rows join the buffer afterwards).
     if (use_sur && nbuf >= 32 && P->train_steps > 0) {
@@ -659,7 +694,7 @@ extern "C" int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa
       }
       t_train += now_s() - tt;
     }
-    rc = gg_plan_wait(plan.p, res.data());
+    if (!incr.p) rc = gg_plan_wait(plan.p, res.data());
     t_exact += now_s() - te;
     if (rc) return rc;
     evals += ncand;
@@ -744,7 +779,20 @@ extern "C" int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa
           }
         }
         moves_acc += C.nm;
-        if ((rc = gg_plan_set_base(plan.p, g.adj.data()))) return rc;
+        if (incr.p) {
+          const double ti = now_s();
+          if (!incr_pending) {
+            int32_t rem[4], add[4];
+            gg_result r2;
+            move_edges(props[C.prop[0]], rem, add);
+            if ((rc = gg_incr_eval(incr.p, rem, add, &r2, nullptr))) return rc;
+          }
+          if ((rc = gg_incr_accept(incr.p))) return rc;
+          incr_pending = false;
+          t_exact += now_s() - ti;
+        } else if ((rc = gg_plan_set_base(plan.p, g.adj.data()))) {
+          return rc;
+        }
         cur_diam = nd;
         cur_S = nS;
         ++accepts;
@@ -764,6 +812,10 @@ extern "C" int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa
           if (rc) return rc;
         }
       }
+    }
+    if (incr_pending) {  // not accepted
+      if ((rc = gg_incr_reject(incr.p))) return rc;
+      incr_pending = false;
     }
     if (!explore) m_cur = improved ? std::min(P->combine, m_cur + 1) : std::max(1, m_cur / 2);
     const double best_aspl = (double)best_S / npairs;

@@ -184,7 +184,8 @@ class BsaParams(ctypes.Structure):
                 ("train_steps", ctypes.c_int32), ("batch_train", ctypes.c_int32),
                 ("lr", ctypes.c_float), ("hist_maxd", ctypes.c_int32),
                 ("feature_refresh", ctypes.c_int32), ("log_capacity", ctypes.c_int32),
-                ("grid_height", ctypes.c_int32), ("grid_length", ctypes.c_int32)]
+                ("grid_height", ctypes.c_int32), ("grid_length", ctypes.c_int32),
+                ("incremental", ctypes.c_int32), ("pad", ctypes.c_int32)]
 
 
 class BsaStats(ctypes.Structure):
@@ -217,7 +218,7 @@ def sa_batched(edges, nodes: int, degree: int, *, max_steps: int = 100, time_bud
                train_steps: int = 1, batch_train: int = 256, lr: float = 1e-3, hist_maxd: int = 8,
                feature_refresh: int = 100, log_capacity: int = 100000,
                accepted_capacity: int = 100000, grid_height: int = 0,
-               grid_length: int = 0) -> BatchedResult:
+               grid_length: int = 0, incremental: bool = False) -> BatchedResult:
     """Batched (and, with use_surrogate, MLP-ranked) SA on the GPU; see
     include/gg_host.h.  exact = 1 without the surrogate is plain single-graph
     GPU SA with a device-resident adjacency.  grid_height > 0 keeps every
@@ -238,7 +239,7 @@ def sa_batched(edges, nodes: int, degree: int, *, max_steps: int = 100, time_bud
     p = BsaParams(max_steps, time_budget_s, target_aspl, proposals, exact, combine, warmup,
                   explore_every, int(use_surrogate), int(random_ranking), max_temp, min_temp,
                   seed, train_steps, batch_train, lr, hist_maxd, feature_refresh, log_capacity,
-                  grid_height, grid_length)
+                  grid_height, grid_length, int(incremental))
     st = BsaStats()
     lt = np.zeros(max(log_capacity, 1))
     la = np.zeros(max(log_capacity, 1))

@@ -160,3 +160,23 @@ def test_gpu_batched_grid_sa_respects_length_and_is_exact(surrogate):
         SA.apply_move(e, t0, t1, v)
         x, y = e // h, e % h
         assert np.all(np.abs(x[:, 0] - x[:, 1]) + np.abs(y[:, 0] - y[:, 1]) <= r)
+
+
+@pytest.mark.parametrize("n,d,exact,grid", [(4096, 16, 1, 0), (2048, 8, 4, 0), (900, 4, 1, 30)])
+def test_gpu_plain_sa_incremental_is_the_same_run(n, d, exact, grid):
+    """Plain batched SA scored by the incremental scorer makes the same run:
+    the same accepted moves with the same exact (diameter, S), the same best
+    graph -- general graphs (one and four candidates per step) and a grid."""
+    print(f"plain sa incremental n={n} d={d} exact={exact} grid={grid}", flush=True)
+    e0 = G.random_grid(30, 30, 4, 2, 3) if grid else G.random_regular(n, d, 7)
+    kw = dict(max_steps=200, exact=exact, use_surrogate=False, seed=4, max_temp=20.0,
+              min_temp=0.5, grid_height=grid, grid_length=2 if grid else 0)
+    full = SA.sa_batched(e0, n, d, **kw)
+    inc = SA.sa_batched(e0, n, d, incremental=True, **kw)
+    assert full.stats["accepts"] > 0
+    assert np.array_equal(full.accepted, inc.accepted)
+    assert np.array_equal(full.accepted_diam, inc.accepted_diam)
+    assert np.array_equal(full.accepted_sum, inc.accepted_sum)
+    assert np.array_equal(full.edges, inc.edges)
+    assert full.stats["best_aspl"] == inc.stats["best_aspl"]
+    assert full.stats["evaluations"] == inc.stats["evaluations"]
