@@ -51,6 +51,7 @@ namespace {
 constexpr int kProbes = 4;
 constexpr int kWord = 16;         // sources per unit (uint16 reach words)
 constexpr int kLocalHist = 64;    // eccentricities histogrammed in LDS by the flag kernel
+constexpr double kInlineSrc = 0.3;
 
 // The moved graph's result, written by the finalize kernel.
 struct IncrResult {
@@ -209,13 +210,27 @@ gg_incr_commit_kernel(const int32_t *__restrict__ aff_count, const uint16_t *__r
                       const uint32_t *__restrict__ new_sum, const uint16_t *__restrict__ new_ecc,
                       uint32_t *__restrict__ src_sum, uint16_t *__restrict__ src_ecc,
                       int32_t *__restrict__ hist) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= *aff_count) return;
-  const int s = aff[i];
-  atomicSub(&hist[src_ecc[s]], 1);
-  atomicAdd(&hist[new_ecc[i]], 1);
-  src_ecc[s] = new_ecc[i];
-  src_sum[s] = new_sum[i];
+  // Histogram changes are gathered per block in LDS (most eccentricities
+  // share a few values), then added with one atomic per touched bin.
+  __shared__ int32_t dh[kLocalHist];
+  const int tid = threadIdx.x;
+  if (tid < kLocalHist) dh[tid] = 0;
+  __syncthreads();
+  const int i = blockIdx.x * blockDim.x + tid;
+  if (i < *aff_count) {
+    const int s = aff[i];
+    const int eo = src_ecc[s], en = new_ecc[i];
+    if (eo != en) {
+      if (eo < kLocalHist) atomicSub(&dh[eo], 1);
+      else atomicSub(&hist[eo], 1);
+      if (en < kLocalHist) atomicAdd(&dh[en], 1);
+      else atomicAdd(&hist[en], 1);
+    }
+    src_ecc[s] = (uint16_t)en;
+    src_sum[s] = new_sum[i];
+  }
+  __syncthreads();
+  if (tid < kLocalHist && dh[tid]) atomicAdd(&hist[tid], dh[tid]);
 }
 
 typedef void (*UnitKernel)(const GraphDev *, const UnitDesc *, UnitOut *, int, int, int);
@@ -293,8 +308,13 @@ struct gg_incr {
   // Current graph.
   uint64_t sum = 0;
   int diameter = 0;
+  // Accepted fraction of scored moves (moving average).  Above kInlineSrc
+  // the per-source values are computed while scoring (a slower list run)
+  // instead of by a second run on accept.
+  double accept_rate = 0.5;
   // Pending move.
   bool pending = false;
+  bool pend_has_src = false;
   Patch undo{}, redo{};
   int32_t pend_rows[4], pend_cols[4], pend_vals[4], pend_old[4];
   IncrResult pend{};
@@ -577,8 +597,10 @@ int gg_incr_eval(gg_incr *s, const int32_t rem[4], const int32_t add[4], gg_resu
                      (const uint16_t *)s->l0, (const uint16_t *)s->l1, mv,
                      (const uint32_t *)s->src_sum, (const uint16_t *)s->src_ecc, s->aff,
                      s->aff_count, s->aff_sum_old, s->removed_hist);
-  hipLaunchKernelGGL(s->K.list, dim3(s->grid), dim3(s->block), s->lds, st,
-                     (const GraphDev *)(s->gdesc + 1), s->outs, s->n_cap, s->qcap);
+  const bool inline_src = s->accept_rate > kInlineSrc;
+  hipLaunchKernelGGL(inline_src ? s->K.src : s->K.list, dim3(s->grid), dim3(s->block), s->lds, st,
+                     (const GraphDev *)(s->gdesc + (inline_src ? 2 : 1)), s->outs, s->n_cap,
+                     s->qcap);
   hipLaunchKernelGGL(gg_incr_finalize_kernel, dim3(1), dim3(256), 0, st, n,
                      (const int32_t *)s->aff_count, (const UnitOut *)s->outs,
                      (const unsigned long long *)s->aff_sum_old, (const int32_t *)s->hist,
@@ -588,6 +610,7 @@ int gg_incr_eval(gg_incr *s, const int32_t rem[4], const int32_t add[4], gg_resu
   GG_HIP(hipMemcpyAsync(s->res_h, s->res, sizeof(IncrResult), hipMemcpyDeviceToHost, st));
   GG_HIP(hipStreamSynchronize(st));
   s->pend = *s->res_h;
+  s->pend_has_src = inline_src;
   s->redo = redo;
   s->undo = undo;
   s->pending = true;
@@ -605,8 +628,10 @@ int gg_incr_accept(gg_incr *s) {
   if (!s->pending) return set_error(GG_ERR_STATE, "no pending move");
   if (!s->pend.connected) return set_error(GG_ERR_STATE, "the pending move disconnects the graph");
   hipStream_t st = s->stream;
-  int rc = run_src(s, 2);
-  if (rc) return rc;
+  if (!s->pend_has_src) {
+    const int rc = run_src(s, 2);
+    if (rc) return rc;
+  }
   hipLaunchKernelGGL(gg_incr_commit_kernel, dim3((s->n + 255) / 256), dim3(256), 0, st,
                      (const int32_t *)s->aff_count, (const uint16_t *)s->aff,
                      (const uint32_t *)s->new_sum, (const uint16_t *)s->new_ecc, s->src_sum,
@@ -617,6 +642,7 @@ int gg_incr_accept(gg_incr *s) {
   s->sum = s->pend.sum;
   s->diameter = s->pend.diameter;
   s->pending = false;
+  s->accept_rate = 0.95 * s->accept_rate + 0.05;
   return GG_OK;
 }
 
@@ -626,6 +652,7 @@ int gg_incr_reject(gg_incr *s) {
   hipLaunchKernelGGL(gg_incr_patch_kernel, dim3(1), dim3(64), 0, s->stream, s->adjR, s->undo);
   GG_HIP(hipGetLastError());
   s->pending = false;
+  s->accept_rate *= 0.95;
   return GG_OK;
 }
 
