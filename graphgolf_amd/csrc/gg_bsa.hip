@@ -31,7 +31,7 @@ using gg::set_error;
 namespace {
 constexpr int kFeat = 64;        // MLP input width
 constexpr int kVStat = 10;       // per-vertex statistics (see VertexStats)
-constexpr int kMaxCombine = 16;  // moves per candidate (4 patch entries each)
+constexpr int kMaxCombine = 128;  // moves per candidate (4 patch entries each)
 constexpr int kBlock = 256;
 }  // namespace
 

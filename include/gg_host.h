@@ -192,7 +192,7 @@ typedef struct gg_bsa_params {
   double target_aspl;     /* stop once best ASPL <= target (0: none) */
   int32_t proposals;      /* moves ranked per ranked step */
   int32_t exact;          /* candidate graphs scored per step */
-  int32_t combine;        /* max moves merged into one candidate (1..16) */
+  int32_t combine;        /* max moves merged into one candidate (1..128) */
   int32_t warmup;         /* initial explore steps */
   int32_t explore_every;  /* every k-th later step explores (0: never) */
   int32_t use_surrogate;

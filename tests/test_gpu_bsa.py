@@ -125,12 +125,15 @@ def test_gpu_surrogate_gain_over_plain_and_random_ranking():
     assert gains["mlp"] > 2 * gains["random"], gains
 
 
-def test_gpu_surrogate_sa_full_size_accepts_are_exact():
+@pytest.mark.parametrize("proposals,combine", [(4096, 16), (12288, 128)])
+def test_gpu_surrogate_sa_full_size_accepts_are_exact(proposals, combine):
     """n = 65536, d = 16: every accepted candidate re-scored by the unpatched
-    evaluator after replaying the logged moves; the best graph by the oracle."""
+    evaluator after replaying the logged moves; the best graph by the oracle
+    (candidates of up to 16 and up to 128 merged moves)."""
     n, d = 65536, 16
+    print(f"full-size surrogate SA proposals={proposals} combine={combine}", flush=True)
     e0 = G.random_regular(n, d, 5)
-    res = SA.sa_batched(e0, n, d, max_steps=100, proposals=4096, combine=16, warmup=30,
+    res = SA.sa_batched(e0, n, d, max_steps=100, proposals=proposals, combine=combine, warmup=30,
                         explore_every=8, seed=4)
     assert res.stats["moves_accepted"] > res.stats["accepts"] > 0
     e = e0.copy()
