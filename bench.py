@@ -39,6 +39,7 @@ WORKLOADS = {
     "c1": "apsp n=256 d=8 g=1 (random regular, seed 1)",
     "c4": "apsp grid 100x100 d=4 r=2 (seed 1)",
     "c3": "batch of 512 graphs n~U[1024,16384] d in {8,12,16,24,32} (one 2-opt each)",
+    "c5i": "incremental scoring of random 2-opt moves, n=65536 d=16 (random regular, seed 5)",
     "c4b": "batch of 256 length-respecting 2-opt proposals of the grid 100x100 d=4 r=2 (seed 1)",
 }
 
@@ -147,6 +148,65 @@ def sa_time_to_aspl(budget_s):
             "speedup": round(t_plain / t_sur, 2) if t_sur > 0 else None}
 
 
+def incremental_bench(args):
+    """A step = one random 2-opt move of the C5 graph scored by the
+    incremental scorer and dropped (the current graph stays the same)."""
+    from graphgolf_amd import graphs as G
+    from graphgolf_amd.incr import IncrementalScorer
+
+    n, d = 65536, 16
+    adj = G.edges_to_adj(n, d, G.random_regular(n, d, 5))
+    rng = np.random.default_rng(0)
+
+    def move():
+        while True:
+            a = int(rng.integers(n))
+            b = int(adj[a, rng.integers(d)])
+            c = int(rng.integers(n))
+            e = int(adj[c, rng.integers(d)])
+            if len({a, b, c, e}) < 4:
+                continue
+            add = (a, e, c, b) if rng.integers(2) == 0 else (a, c, b, e)
+            if add[1] in adj[add[0]] or add[3] in adj[add[2]]:
+                continue
+            return (a, b, c, e), add
+
+    sc = IncrementalScorer(adj)
+    moves = [move() for _ in range(args.warmup + args.steps)]
+    for rem, add in moves[:args.warmup]:
+        sc.score(rem, add)
+        sc.reject()
+    affected, diam = [], []
+    t0 = time.perf_counter()
+    for rem, add in moves[args.warmup:]:
+        r, a = sc.score(rem, add)
+        sc.reject()
+        affected.append(a)
+        diam.append(r.diameter)
+    elapsed = time.perf_counter() - t0
+    ms = 1000.0 * elapsed / args.steps
+    # Algorithmic bytes of re-running the affected sources (the single-tile
+    # formula with w = affected/64 words), per scored move.
+    k = float(np.mean(affected))
+    bytes_move = float(np.mean(diam)) * n * (4 * d + 8 * (k / 64.0) * (d + 2))
+    achieved = bytes_move / (ms * 1e-3) / 1e9
+    line = {
+        "metric": "APSP evals/sec + achieved GB/s; SA time-to-ASPL, n=65536 d=16 on 1 GPU",
+        "value": round(1000.0 / ms, 3), "unit": "scored moves/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u16",
+        "data": "synthetic",
+        "config": {"workload": WORKLOADS["c5i"], "nodes": n, "degree": d,
+                   "affected_mean": round(k, 1), "affected_fraction": round(k / n, 4),
+                   "full_evaluation_ms": None},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                     "time": "whole scored move (probe, flag, list run, finalize, host sync)"},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def read_traffic(workload):
     """HBM bytes per tile-BFS launch from the committed rocprofv3 PMC summary."""
     path = os.path.join(ROOT, "profiles", f"traffic_{workload}.md")
@@ -178,6 +238,10 @@ def main():
 
     from graphgolf_amd import evaluation as E
 
+    if args.workload == "c5i":
+        E.init(0)
+        incremental_bench(args)
+        return
     items = build_workload(args.workload)
     E.init(0)
     if args.word_bits:
