@@ -157,6 +157,79 @@ using gg::Rng;
 
 // A 2-opt move on edge-list lines t0 = (a,b), t1 = (c,d):
 // r = 0 -> (a,d), (c,b);  r = 1 -> (a,c), (b,d).
+// ---------------------------------------------------------------------------
+// Move generation on the device (ranked steps of general graphs).
+
+// splitmix64 finaliser: a counter-based stream, one value per (seed, step,
+// proposal, draw).
+__device__ __forceinline__ uint64_t bsa_mix(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+__device__ __forceinline__ bool row_has(const int32_t *__restrict__ adj, int d, int v, int u) {
+  bool hit = false;
+  for (int j = 0; j < d; ++j) hit |= adj[(size_t)v * d + j] == u;
+  return hit;
+}
+
+// Thread i draws lines t0 != t1 and a variant r (r = 0 adds (a,d),(c,b);
+// r = 1 adds (a,c),(b,d), as exchange_edge_2opt's two choices,
+// general/sa.c:254-266), redrawing while the two lines share a vertex
+// (has_duplicated_vertex, general/common.c:119-122), and marks the move
+// valid when neither added edge exists in the current adjacency (the
+// duplicate check, general/common.c:184-276, as d-entry row scans).
+// out[i] = (a, b, c, d, r) for the feature kernel; lines_out[i] = (t0, t1,
+// valid).
+__global__ void __launch_bounds__(256)
+gg_bsa_propose(int count, uint64_t seed, uint64_t step, int lines, const int32_t *__restrict__ edges,
+               const int32_t *__restrict__ adj, int d, int32_t *__restrict__ out,
+               int32_t *__restrict__ lines_out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const uint64_t base = bsa_mix(seed ^ bsa_mix(step * 0x100000001B3ull + (uint64_t)i));
+  int t0 = 0, t1 = 0, r = 0, a = 0, b = 0, c = 0, e = 0;
+  bool ok = false;
+  for (int k = 0; k < 64 && !ok; ++k) {
+    const uint64_t x = bsa_mix(base + (uint64_t)k);
+    t0 = (int)(((x & 0xffffffffull) * (uint64_t)lines) >> 32);
+    t1 = (int)(((x >> 32) * (uint64_t)lines) >> 32);
+    r = (int)((bsa_mix(x) >> 63) & 1u);
+    a = edges[2 * t0];
+    b = edges[2 * t0 + 1];
+    c = edges[2 * t1];
+    e = edges[2 * t1 + 1];
+    ok = t0 != t1 && a != c && a != e && b != c && b != e;
+  }
+  bool valid = ok;
+  if (ok) {
+    const int q1 = r ? c : e, p2 = r ? b : c, q2 = r ? e : b;
+    valid = !row_has(adj, d, a, q1) && !row_has(adj, d, p2, q2);
+  }
+  int32_t *o = out + 5 * (size_t)i;
+  o[0] = a;
+  o[1] = b;
+  o[2] = c;
+  o[3] = e;
+  o[4] = r;
+  int32_t *l = lines_out + 3 * (size_t)i;
+  l[0] = t0;
+  l[1] = t1;
+  l[2] = valid ? 1 : 0;
+}
+
+// Writes changed lines (line, u, v) into the device edge list.
+__global__ void __launch_bounds__(256)
+gg_bsa_set_lines(const int32_t *__restrict__ upd, int count, int32_t *__restrict__ edges) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const int t = upd[3 * i];
+  edges[2 * t] = upd[3 * i + 1];
+  edges[2 * t + 1] = upd[3 * i + 2];
+}
+
 struct Prop {
   int t0, t1, r, a, b, c, d;
 };
@@ -376,7 +449,7 @@ template <class T>
 struct DevArr {
   T *p = nullptr;
   ~DevArr() {
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
   }
   int alloc(size_t count) {
     if (hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess)
@@ -409,6 +482,32 @@ extern "C" int gg_apply_move(int lines, int32_t *edges, int line0, int line1, in
   edges[2 * line0 + 1] = variant ? c : d;
   edges[2 * line1] = variant ? b : c;
   edges[2 * line1 + 1] = variant ? d : b;
+  return GG_OK;
+}
+
+// The device proposal draw of ranked steps, for tests: `count` proposals of
+// the graph (edges [lines][2], adjacency [n][d]) for (seed, step).
+extern "C" int gg_sa_proposals(int nodes, int degree, const int32_t *edges, const int32_t *adj,
+                               int count, uint64_t seed, uint64_t step, int32_t *props,
+                               int32_t *lines_out) {
+  if (!edges || !adj || !props || !lines_out || nodes < 4 || degree < 1 || count < 1)
+    return set_error(GG_ERR_INVALID, "bad proposal arguments");
+  int rc = gg_init(-1);
+  if (rc) return rc;
+  const int lines = nodes * degree / 2;
+  DevArr<int32_t> e_d, a_d, p_d, l_d;
+  if ((rc = e_d.alloc(2 * (size_t)lines)) || (rc = a_d.alloc((size_t)nodes * degree)) ||
+      (rc = p_d.alloc((size_t)count * 5)) || (rc = l_d.alloc((size_t)count * 3)))
+    return rc;
+  hipStream_t es = gg::engine_stream();
+  BSA_HIP(hipMemcpyAsync(e_d.p, edges, 2 * (size_t)lines * 4, hipMemcpyHostToDevice, es));
+  BSA_HIP(hipMemcpyAsync(a_d.p, adj, (size_t)nodes * degree * 4, hipMemcpyHostToDevice, es));
+  hipLaunchKernelGGL(gg_bsa_propose, dim3((count + 255) / 256), dim3(256), 0, es, count, seed, step,
+                     lines, (const int32_t *)e_d.p, (const int32_t *)a_d.p, degree, p_d.p, l_d.p);
+  BSA_HIP(hipGetLastError());
+  BSA_HIP(hipMemcpyAsync(props, p_d.p, (size_t)count * 5 * 4, hipMemcpyDeviceToHost, es));
+  BSA_HIP(hipMemcpyAsync(lines_out, l_d.p, (size_t)count * 3 * 4, hipMemcpyDeviceToHost, es));
+  BSA_HIP(hipStreamSynchronize(es));
   return GG_OK;
 }
 
@@ -527,7 +626,15 @@ extern "C" int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa
   vs.maxd = std::min(std::max(P->hist_maxd, 2), 15);
   const int cap = 16384;
   const int bt = std::max(P->batch_train, 32);
-  DevArr<int32_t> props_d, idx_d, tidx_d;
+  DevArr<int32_t> props_d, idx_d, tidx_d, edges_d, plines_d, upd_d;
+  struct Pinned {
+    int32_t *props = nullptr, *lines = nullptr, *upd = nullptr;
+    ~Pinned() {
+      if (props) (void)hipHostFree(props);
+      if (lines) (void)hipHostFree(lines);
+      if (upd) (void)hipHostFree(upd);
+    }
+  } pin;
   DevArr<float> vstat_d, feats_d, replay_d, batch_d;
   std::vector<double> by(cap);  // raw labels of the replay rows
   int nbuf = 0, head = 0;
@@ -540,8 +647,16 @@ extern "C" int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa
     if ((rc = props_d.alloc((size_t)npmax * 5)) || (rc = feats_d.alloc((size_t)npmax * kFeat)) ||
         (rc = replay_d.alloc((size_t)cap * kFeat)) || (rc = batch_d.alloc((size_t)bt * kFeat)) ||
         (rc = vstat_d.alloc((size_t)nodes * kVStat)) ||
-        (rc = idx_d.alloc(2 * (size_t)std::max(npmax, bt))) || (rc = tidx_d.alloc((size_t)bt)))
+        (rc = idx_d.alloc(2 * (size_t)std::max(npmax, bt))) || (rc = tidx_d.alloc((size_t)bt)) ||
+        (rc = edges_d.alloc(2 * (size_t)g.lines)) || (rc = plines_d.alloc((size_t)npmax * 3)) ||
+        (rc = upd_d.alloc(3 * 2 * (size_t)kMaxCombine)))
       return rc;
+    if (hipHostMalloc((void **)&pin.props, (size_t)npmax * 5 * 4) != hipSuccess ||
+        hipHostMalloc((void **)&pin.lines, (size_t)npmax * 3 * 4) != hipSuccess ||
+        hipHostMalloc((void **)&pin.upd, 3 * 2 * (size_t)kMaxCombine * 4) != hipSuccess)
+      return set_error(GG_ERR_HIP, "pinned allocation failed");
+    if (hipMemcpy(edges_d.p, edges, 2 * (size_t)g.lines * 4, hipMemcpyHostToDevice) != hipSuccess)
+      return set_error(GG_ERR_HIP, "edge list upload failed");
     const double tf = now_s();
     rc = vs.refresh(g);
     if (!rc && hipMemcpy(vstat_d.p, vs.host.data(), vs.host.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
@@ -573,7 +688,9 @@ extern "C" int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa
     // --- proposals
     double tp = now_s();
     const int np = explore ? E : NP;
-    for (int i = 0; i < np; ++i) {
+    // Ranked steps of general graphs draw their proposals on the device.
+    const bool dev_props = use_sur && !explore && g.grid_height == 0;
+    for (int i = 0; i < (dev_props ? 0 : np); ++i) {
       Prop &p = props[i];
       if (explore) {
         Slots s;
@@ -598,13 +715,41 @@ extern "C" int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa
     // --- features + ranking on the device
     if (use_sur) {
       const double ts = now_s();
-      BSA_HIP(hipMemcpyAsync(props_d.p, props_h.data(), (size_t)np * 5 * 4, hipMemcpyHostToDevice, es));
+      if (dev_props) {
+        hipLaunchKernelGGL(gg_bsa_propose, dim3((np + 255) / 256), dim3(256), 0, es, np, P->seed,
+                           (uint64_t)step, g.lines, (const int32_t *)edges_d.p,
+                           (const int32_t *)gg_plan_base_device(plan.p), degree, props_d.p,
+                           plines_d.p);
+        BSA_HIP(hipGetLastError());
+      } else {
+        BSA_HIP(hipMemcpyAsync(props_d.p, props_h.data(), (size_t)np * 5 * 4, hipMemcpyHostToDevice, es));
+      }
       hipLaunchKernelGGL(gg_bsa_features, dim3(np), dim3(kBlock), feat_lds, es,
                          (const int32_t *)gg_plan_base_device(plan.p), nodes, degree,
                          (const int32_t *)props_d.p, (const float *)vstat_d.p, feats_d.p);
       BSA_HIP(hipGetLastError());
       BSA_HIP(hipStreamSynchronize(es));
       if ((rc = gg::mlp_forward_device(mlp.p, feats_d.p, np, pred.data()))) return rc;
+      if (dev_props) {
+        // The drawn moves come back for candidate assembly; invalid ones
+        // (no vertex-disjoint pair in 64 draws, or an added edge that
+        // exists) rank last.
+        BSA_HIP(hipMemcpyAsync(pin.props, props_d.p, (size_t)np * 5 * 4, hipMemcpyDeviceToHost, es));
+        BSA_HIP(hipMemcpyAsync(pin.lines, plines_d.p, (size_t)np * 3 * 4, hipMemcpyDeviceToHost, es));
+        BSA_HIP(hipStreamSynchronize(es));
+        for (int i = 0; i < np; ++i) {
+          Prop &p = props[i];
+          const int32_t *o = pin.props + 5 * (size_t)i, *l = pin.lines + 3 * (size_t)i;
+          p.a = o[0];
+          p.b = o[1];
+          p.c = o[2];
+          p.d = o[3];
+          p.r = o[4];
+          p.t0 = l[0];
+          p.t1 = l[1];
+          if (!l[2]) pred[i] = 1e30f;
+        }
+      }
       t_sur += now_s() - ts;
     }
     std::vector<float> &rank = (use_sur && P->random_ranking && !explore) ? noise : pred;
@@ -779,6 +924,22 @@ extern "C" int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa
           }
         }
         moves_acc += C.nm;
+        if (use_sur) {  // the device edge list follows the accepted moves
+          int nu = 0;
+          for (int k = 0; k < C.nm; ++k) {
+            const Prop &p = props[C.prop[k]];
+            for (int t : {p.t0, p.t1}) {
+              pin.upd[3 * nu] = t;
+              pin.upd[3 * nu + 1] = edges[2 * t];
+              pin.upd[3 * nu + 2] = edges[2 * t + 1];
+              ++nu;
+            }
+          }
+          BSA_HIP(hipMemcpyAsync(upd_d.p, pin.upd, (size_t)nu * 3 * 4, hipMemcpyHostToDevice, es));
+          hipLaunchKernelGGL(gg_bsa_set_lines, dim3((nu + 255) / 256), dim3(256), 0, es,
+                             (const int32_t *)upd_d.p, nu, edges_d.p);
+          BSA_HIP(hipGetLastError());
+        }
         if (incr.p) {
           const double ti = now_s();
           if (!incr_pending) {

@@ -283,3 +283,20 @@ def sa_features(adj: np.ndarray, props: np.ndarray, vstat: np.ndarray | None = N
     L.check(lib.gg_sa_features(n, d, L.ptr(adj), props.shape[0], L.ptr(props), vs,
                                out.ctypes.data_as(fp)))
     return out
+
+
+def sa_proposals(edges, nodes: int, degree: int, count: int, seed: int, step: int):
+    """The device move draw of ranked batched-SA steps: (props [count, 5] =
+    (a, b, c, d, r), lines [count, 3] = (t0, t1, valid))."""
+    lib = L.lib()
+    lib.gg_sa_proposals.argtypes = [ctypes.c_int, ctypes.c_int, L.i32p, L.i32p, ctypes.c_int,
+                                    ctypes.c_uint64, ctypes.c_uint64, L.i32p, L.i32p]
+    lib.gg_sa_proposals.restype = ctypes.c_int
+    from .graphs import edges_to_adj
+    e = L.c32(edges).reshape(-1, 2)
+    adj = edges_to_adj(nodes, degree, e)
+    props = np.zeros((count, 5), dtype=np.int32)
+    lines = np.zeros((count, 3), dtype=np.int32)
+    L.check(lib.gg_sa_proposals(nodes, degree, L.ptr(e), L.ptr(adj), count, seed, step,
+                                L.ptr(props), L.ptr(lines)))
+    return props, lines

@@ -248,6 +248,15 @@ int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa_params *pa
 int gg_sa_features(int nodes, int degree, const int32_t *adj, int count, const int32_t *props,
                    const float *vstat, float *feats);
 
+/* Ranked steps of general graphs draw their proposals on the device: lines
+ * t0 != t1 and a variant r from a counter-based stream of (seed, step,
+ * proposal), redrawn (up to 64 times) while the lines share a vertex, valid
+ * when neither added edge exists.  props[i] = (a, b, c, d, r) with (a, b) =
+ * line t0 and (c, d) = line t1; lines_out[i] = (t0, t1, valid).  Exposed for
+ * tests. */
+int gg_sa_proposals(int nodes, int degree, const int32_t *edges, const int32_t *adj, int count,
+                    uint64_t seed, uint64_t step, int32_t *props, int32_t *lines_out);
+
 /* Applies one logged move (line0, line1, variant) to an edge list the way
  * gg_sa_batched does (for replays). */
 int gg_apply_move(int lines, int32_t *edges, int line0, int line1, int variant);

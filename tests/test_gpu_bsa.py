@@ -183,3 +183,35 @@ def test_gpu_plain_sa_incremental_is_the_same_run(n, d, exact, grid):
     assert np.array_equal(full.edges, inc.edges)
     assert full.stats["best_aspl"] == inc.stats["best_aspl"]
     assert full.stats["evaluations"] == inc.stats["evaluations"]
+
+
+@pytest.mark.parametrize("n,d", [(64, 3), (4096, 16), (65536, 16)])
+def test_gpu_device_proposals_are_valid_two_opt_moves(n, d):
+    """The device draw: (a,b) and (c,d) are the drawn lines, the four
+    endpoints distinct, `valid` exactly when neither added edge exists, lines
+    and variants spread evenly; the same (seed, step) gives the same draw."""
+    print(f"device proposals n={n} d={d}", flush=True)
+    e = G.random_regular(n, d, 9)
+    count = 20000
+    props, lines = SA.sa_proposals(e, n, d, count, seed=3, step=7)
+    t0, t1, valid = lines[:, 0], lines[:, 1], lines[:, 2]
+    a, b, c, dd, r = props.T
+    assert np.array_equal(e[t0, 0], a) and np.array_equal(e[t0, 1], b)
+    assert np.array_equal(e[t1, 0], c) and np.array_equal(e[t1, 1], dd)
+    assert np.all(t0 != t1)
+    assert np.all((a != c) & (a != dd) & (b != c) & (b != dd))
+    keys = set(map(tuple, np.sort(e, axis=1).tolist()))
+    q1 = np.where(r == 1, c, dd)
+    p2 = np.where(r == 1, b, c)
+    q2 = np.where(r == 1, dd, b)
+    exists = np.array([(min(x, y), max(x, y)) in keys or (min(u, v), max(u, v)) in keys
+                       for x, y, u, v in zip(a, q1, p2, q2)])
+    assert np.array_equal(valid == 1, ~exists)
+    assert 0.45 < r.mean() < 0.55
+    lines_n = len(e)
+    hist = np.bincount(np.concatenate([t0, t1]) * 16 // lines_n, minlength=16)
+    assert hist.min() > 0.8 * hist.mean()
+    props2, lines2 = SA.sa_proposals(e, n, d, count, seed=3, step=7)
+    assert np.array_equal(props, props2) and np.array_equal(lines, lines2)
+    props3, _ = SA.sa_proposals(e, n, d, count, seed=3, step=8)
+    assert not np.array_equal(props, props3)
