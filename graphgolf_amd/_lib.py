@@ -42,6 +42,11 @@ class GGResult(ctypes.Structure):
                 ("sum", ctypes.c_uint64)]
 
 
+class GGGeneralShape(ctypes.Structure):
+    _fields_ = [("nodes", ctypes.c_int32), ("degree", ctypes.c_int32), ("groups", ctypes.c_int32),
+                ("added_centers", ctypes.c_int32)]
+
+
 class GGSaParams(ctypes.Structure):
     _fields_ = [("ncalcs", ctypes.c_int64), ("max_temp", ctypes.c_double),
                 ("min_temp", ctypes.c_double), ("cooling_cycle", ctypes.c_int32),
@@ -61,6 +66,9 @@ EVAL_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c
                            i32p, u64p, i32p)
 EVAL_GRID_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                 ctypes.c_int, ctypes.c_int, i32p, i32p, i32p, u64p, i32p)
+# (ctx, nodes, degree, based_nodes, groups, added_centers, adj, diameter, sum, connected)
+EVAL_SYM_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                               ctypes.c_int, ctypes.c_int, ctypes.c_int, i32p, i32p, u64p, i32p)
 
 
 class GGGridShape(ctypes.Structure):
@@ -135,6 +143,12 @@ def lib():
                                 ctypes.POINTER(GGSaParams), EVAL_FN, ctypes.c_void_p,
                                 ctypes.POINTER(GGSaStats), i32p, u64p], ctypes.c_int),
         "gg_grid_lattice": ([ctypes.c_int] * 4 + [ctypes.c_uint32, i32p], ctypes.c_int),
+        "gg_sa_general_sym": ([ctypes.POINTER(GGGeneralShape), i32p, ctypes.POINTER(GGSaParams),
+                               EVAL_SYM_FN, ctypes.c_void_p, ctypes.POINTER(GGSaStats), i32p,
+                               u64p], ctypes.c_int),
+        "gg_general_program_sym": ([ctypes.POINTER(GGGeneralShape), i32p, ctypes.c_int, ctypes.c_int,
+                                    ctypes.POINTER(GGSaParams), EVAL_SYM_FN, ctypes.c_void_p,
+                                    ctypes.POINTER(GGSaStats), i32p, u64p], ctypes.c_int),
         "gg_incr_create": ([ctypes.c_int, ctypes.c_int, i32p], ctypes.c_void_p),
         "gg_incr_state": ([ctypes.c_void_p, ctypes.POINTER(GGResult)], ctypes.c_int),
         "gg_incr_eval": ([ctypes.c_void_p, i32p, i32p, ctypes.POINTER(GGResult), i32p],

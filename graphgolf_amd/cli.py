@@ -3,7 +3,8 @@
 device evaluator.
 
     python -m graphgolf_amd general -f <edge_file> [-o <output_file>] [-s <random_seed>]
-        [-n <num_calculations>] [-w <max_temperature>] [-c <min_temperature>] [-B] [-H] [-L] [-M]
+        [-n <num_calculations>] [-w <max_temperature>] [-c <min_temperature>] [-g <groups>]
+        [-v <added_centers>] [-B] [-H] [-L] [-M]
     python -m graphgolf_amd grid [-f <edge_file>] [-W <width>] [-H <height>] [-D <degree>]
         -R <length> [-o <output_file>] [-s <random_seed>] [-n <calculations>]
         [-w <max_temperature>] [-c <min_temperature>] [-g <groups>] [-C <cooling_cycle>]
@@ -15,8 +16,8 @@ start-graph moves (skipped with -M, which resumes from a whole graph written
 by -o), the same annealing moves, the same output file.  -B and -L choose the
 reference's evaluation algorithm; every algorithm gives the same results and
 this build always runs the device evaluator, so they only change the banner.
-The general program runs groups = 1 without added centres: -g > 1, -v, -D
-and -N stop with a message, as does the grid program's -d.  -O (listed by the
+-D and -N stop with a message (temperature detection, non-simple graphs),
+as does the grid program's -d.  -O (listed by the
 reference's help, absent from its option string) skips the graph checks.
 """
 from __future__ import annotations
@@ -126,12 +127,13 @@ def _results(r, low_diam, low_aspl, elapsed, sep, out):
 
 
 def run_general(argv, evaluator=None, out=sys.stdout, incremental=None):
-    """The general program (general/main.c:317-497), groups = 1.  Returns the
-    SaResult (edges = the best graph).  `evaluator` (adj -> (connected,
-    diameter, S)) replaces the device evaluator when given.  `incremental`
-    (None: from n >= 4096 with degree <= 16 on the device evaluator) scores
-    proposals with the incremental scorer; its results are the same exact
-    integers, so the run and the output file are the same."""
+    """The general program (general/main.c:317-497).  Returns the SaResult
+    (edges = the best graph).  `evaluator` (adj, based_nodes, groups,
+    added_centers -> (connected, diameter, S)) replaces the device evaluator
+    when given.  `incremental` (None: from n >= 4096 with degree <= 16 on the
+    device evaluator; groups = 1 without centres) scores proposals with the
+    incremental scorer; its results are the same exact integers, so the run
+    and the output file are the same."""
     from . import graphs as G
     from . import sa as SA
 
@@ -151,20 +153,45 @@ def run_general(argv, evaluator=None, out=sys.stdout, incremental=None):
         raise CliError("Both -H and -c cannot be used.")
     if hill and "D" in o:
         raise CliError("Both -H and -D cannot be used.")
-    if groups != 1 or centers:
-        raise CliError("this program runs groups = 1 without added centres (-g, -v); the "
-                       "evaluator itself takes both (include/gg_eval.h)")
+    if "D" in o and groups != 1:
+        raise CliError("When using -D option, -g must be 1.")
     if "D" in o or "N" in o:
         raise CliError("-D and -N are not supported by this program")
     infile, outfile = o["f"], o.get("o")
     if not os.path.exists(infile):
         raise CliError("File not found")
     edges = G.read_edges(infile)
-    nodes = int(edges.max()) + 1
-    degree = 2 * len(edges) // nodes
+    # general/main.c:361-422
+    file_lines = len(edges)
+    lines = file_lines if halfway else file_lines * groups
+    based_nodes = int(edges.max()) + 1
+    if halfway:
+        if centers:
+            based_nodes -= centers
+        if based_nodes % groups:
+            raise CliError("based_nodes must be divisible by groups")
+        based_nodes //= groups
+    nodes = based_nodes * groups
+    degree = 2 * lines // nodes
     if nodes <= degree:
         raise CliError(f"n is too small. nodes = {nodes} degree = {degree}")
-    if verify:
+    source = SA.GENERAL_FULL_FILE if halfway else SA.GENERAL_BASE_FILE
+    if not halfway and centers:
+        degree += 1
+        nodes += centers
+        if centers > based_nodes:
+            raise CliError(f"-v value ({centers}) is invalid")
+        if (degree * nodes) % 2:
+            raise CliError(f"Number of degrees or number of nodes is invalid {degree} {nodes}")
+        if (nodes - centers) // groups % 2:
+            raise CliError("Number of nodes, groups, or added_centers is invalid 1")
+        if ((nodes - centers) // groups - degree // groups * centers) % 2:
+            raise CliError("Number of nodes, groups, or added_centers is invalid 2")
+        source = SA.GENERAL_BASE_CENTRES
+    elif halfway and centers:
+        nodes += centers
+    symmetric = groups > 1 or centers > 0
+    if verify and not symmetric:
         verify_regular(nodes, degree, edges, out)
     low_diam, low_aspl = G.moore_bound(nodes, degree)
     rate = 1.0 if hill else (math.pow(min_t / max_t, 1.0 / ncalcs) if max_t != min_t else 1.0)
@@ -179,16 +206,24 @@ def run_general(argv, evaluator=None, out=sys.stdout, incremental=None):
               f"   MIN Temperature: {min_t:f}\n   Cooling Cycle: 1\n"
               f"   Cooling Rate : {rate:f}", file=out)
     print(f"APSP: {algo}\nNum. of Calulations: {ncalcs}\nInput filename: {infile}\n"
-          f"   Vertexes: {nodes}\n   Degree:   {degree}\n   Groups:   1", file=out)
+          f"   Vertexes: {nodes}\n   Degree:   {degree}\n   Groups:   {groups}", file=out)
+    if centers:
+        print(f"   Center points:    {centers}", file=out)
     if outfile:
         print(f"Output filename: {outfile}", file=out)
     print("---", file=out)
     t0 = time.perf_counter()
-    if incremental is None:
-        incremental = evaluator is None and nodes >= 4096 and degree <= 16
-    r = SA.general_program(edges, nodes, degree, seed, ncalcs, halfway=halfway, max_temp=max_t,
-                           min_temp=min_t, hill_climbing=hill, evaluator=evaluator,
-                           incremental=incremental)
+    if symmetric:
+        r = SA.general_program_sym(edges, nodes, degree, groups, centers, source, seed, ncalcs,
+                                   max_temp=max_t, min_temp=min_t, hill_climbing=hill,
+                                   evaluator=evaluator)
+    else:
+        if incremental is None:
+            incremental = evaluator is None and nodes >= 4096 and degree <= 16
+        one = None if evaluator is None else (lambda adj: evaluator(adj, nodes, 1, 0))
+        r = SA.general_program(edges, nodes, degree, seed, ncalcs, halfway=halfway,
+                               max_temp=max_t, min_temp=min_t, hill_climbing=hill,
+                               evaluator=one, incremental=incremental)
     if r.already_optimal:
         print("The input file has already optimum solution.", file=out)
         return r

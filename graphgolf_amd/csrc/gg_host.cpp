@@ -320,8 +320,8 @@ namespace {
 
 enum { kRight = 0, kLeft = 1, kMiddle = 2 };
 
-int ref_random_below(int max) { return (int)(random() * ((double)max) / (1.0 + RAND_MAX)); }
-double ref_uniform() { return ((double)random() + 1.0) / ((double)RAND_MAX + 2.0); }
+using gg::ref_random_below;
+using gg::ref_uniform;
 
 // Orientation of an edge on the ring of vertex ids (no added centres).
 int ring_orientation(int n, int a, int b) {
@@ -337,20 +337,7 @@ int ring_orientation(int n, int a, int b) {
 
 // general/main.c:244-265 (Moore bound).
 void moore(int nodes, int degree, int *low_diam, double *low_aspl) {
-  int diam = -1, n = 1, r = 1;
-  double aspl = 0.0;
-  while (true) {
-    int tmp = n + degree * (int)std::pow(degree - 1, r - 1);
-    if (tmp >= nodes) break;
-    n = tmp;
-    aspl += r * degree * std::pow(degree - 1, r - 1);
-    diam = r++;
-  }
-  diam++;
-  aspl += diam * (nodes - n);
-  aspl /= (nodes - 1);
-  *low_diam = diam;
-  *low_aspl = aspl;
+  gg::moore_bound(nodes, degree, low_diam, low_aspl);
 }
 
 struct Move {  // what a 2-opt changed, for restore
@@ -473,7 +460,13 @@ int sa_general_run(int nodes, int degree, int32_t *edges, const gg_sa_params *P,
     return set_error(GG_ERR_INVALID, "bad SA arguments");
   if (P->incremental && eval)
     return set_error(GG_ERR_INVALID, "incremental scoring runs on the device evaluator only");
-  if (!eval) eval = device_eval;
+  if (!eval) {
+    // Bring the device up before srandom(): its runtime's start-up may draw
+    // from the same libc generator.
+    const int rc = gg_init(-1);
+    if (rc) return rc;
+    eval = device_eval;
+  }
   const double t_start = now_s();
   double t_eval = 0;
   GeneralState S(nodes, degree, edges);
@@ -536,8 +529,8 @@ int sa_general_run(int nodes, int degree, int32_t *edges, const gg_sa_params *P,
                                          : 1.0;  // general/main.c:436-442
   double temp = max_t;
   if (seed_rng) srandom(P->seed);
-  int cur_diam, best_diam;
-  double cur_aspl, best_aspl;
+  int cur_diam = 0, best_diam = 0;
+  double cur_aspl = 0, best_aspl = 0;
   int rc = evaluate(&cur_diam, &cur_aspl, nullptr);
   if (rc < 0) return rc;
   best_diam = cur_diam;
@@ -634,7 +627,10 @@ extern "C" int gg_general_program(int nodes, int degree, int32_t *edges, int hal
   int rc = check_simple_regular(nodes, degree, edges);
   if (rc) return rc;
   const gg_eval_fn user_eval = eval;
-  if (!eval) eval = device_eval;
+  if (!eval) {
+    if ((rc = gg_init(-1))) return rc;  // before srandom(), as in sa_general_run
+    eval = device_eval;
+  }
   const int lines = nodes * degree / 2;
   auto connected = [&](const int32_t *adj, int *diam, double *aspl) -> int {
     int32_t dm = 0, conn = 0;
@@ -661,8 +657,8 @@ extern "C" int gg_general_program(int nodes, int degree, int32_t *edges, int hal
   {  // check_current_edge (general/sa.c:557-572)
     std::vector<int32_t> adj((size_t)nodes * degree);
     if ((rc = gg_edges_to_adj(nodes, lines, degree, edges, adj.data()))) return rc;
-    int diam;
-    double aspl;
+    int diam = 0;
+    double aspl = 0;
     if ((rc = connected(adj.data(), &diam, &aspl)) < 0) return rc;
     if (rc == 0)
       return set_error(GG_ERR_INVALID, "The input file has a node which is never reached by another node.");
@@ -935,7 +931,11 @@ int sa_grid_run(const gg_grid_shape *shape, int degree, int32_t *edges, const in
   }
   for (int i = 0; i < 2 * lines; ++i)
     if (edges[i] < 0 || edges[i] >= nodes) return set_error(GG_ERR_INVALID, "edge endpoint out of range");
-  if (!eval) eval = device_eval_grid;
+  if (!eval) {
+    const int rc0 = gg_init(-1);  // before srandom(): the runtime's start-up may draw from it
+    if (rc0) return rc0;
+    eval = device_eval_grid;
+  }
   const double t_start = now_s();
   double t_eval = 0;
   GridMover M{lines, lines / groups, GridGeom{height, width, groups}, shape->length};
@@ -972,8 +972,8 @@ int sa_grid_run(const gg_grid_shape *shape, int degree, int32_t *edges, const in
   const double rate = P->hill_climbing ? 1.0 : std::pow(min_t / max_t, (double)cycle / P->ncalcs);
   double temp = max_t;
   if (seed_rng) srandom(P->seed);
-  int cur_diam, best_diam;
-  double cur_aspl, best_aspl;
+  int cur_diam = 0, best_diam = 0;
+  double cur_aspl = 0, best_aspl = 0;
   rc = evaluate(edges, &cur_diam, &cur_aspl);
   if (rc < 0) return rc;
   if (rc == 0) return set_error(GG_ERR_INVALID, "the start graph is not connected");
@@ -981,8 +981,8 @@ int sa_grid_run(const gg_grid_shape *shape, int degree, int32_t *edges, const in
   best_aspl = cur_aspl;
   int64_t accepts = 0, ii;
   for (ii = 0; ii < P->ncalcs; ++ii) {
-    int nd;
-    double na;
+    int nd = 0;
+    double na = 0;
     while (true) {
       std::copy(edges, edges + 2 * (size_t)lines, tmp.begin());
       M.exchange(tmp.data());
@@ -1266,7 +1266,12 @@ extern "C" int gg_grid_program(const gg_grid_shape *shape, int degree, int32_t *
   if (((int64_t)nodes * degree) % 2 || nodes % g || (nodes * degree / 2) % g)
     return set_error(GG_ERR_INVALID, "(nodes*degree/2) must be divisible by groups(%d)", g);
   const int lines = nodes * degree / 2;
-  if (!eval) eval = device_eval_grid;
+  const gg_eval_grid_fn user_eval = eval;
+  if (!eval) {
+    const int rc0 = gg_init(-1);  // before srandom(), as in sa_grid_run
+    if (rc0) return rc0;
+    eval = device_eval_grid;
+  }
   std::vector<int32_t> ident;
   if (!rotate_hash) {
     if (g != 1) return set_error(GG_ERR_INVALID, "groups > 1 needs rotate_hash");
@@ -1327,6 +1332,6 @@ extern "C" int gg_grid_program(const gg_grid_shape *shape, int degree, int32_t *
       M.exchange(tmp.data());
     }
   }
-  return sa_grid_run(shape, degree, edges, rotate_hash, P, fixed_temp, eval, ctx, stats, log_diam,
+  return sa_grid_run(shape, degree, edges, rotate_hash, P, fixed_temp, user_eval, ctx, stats, log_diam,
                      log_sum, false);
 }

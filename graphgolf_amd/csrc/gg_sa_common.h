@@ -3,7 +3,9 @@
 #ifndef GG_SA_COMMON_H
 #define GG_SA_COMMON_H
 #include <chrono>
+#include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <vector>
 
 namespace gg {
@@ -82,6 +84,30 @@ inline double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+
+// libc random() draws as the reference makes them: getRandom
+// (general/common.c:44-47) and uniform_rand (general/sa.c:36-39).
+inline int ref_random_below(int max) { return (int)(random() * ((double)max) / (1.0 + RAND_MAX)); }
+inline double ref_uniform() { return ((double)random() + 1.0) / ((double)RAND_MAX + 2.0); }
+
+// Moore bound of diameter and ASPL (lower_bound_of_diam_aspl,
+// general/main.c:244-265).
+inline void moore_bound(int nodes, int degree, int *low_diam, double *low_aspl) {
+  int diam = -1, n = 1, r = 1;
+  double aspl = 0.0;
+  while (true) {
+    const int tmp = n + degree * (int)std::pow(degree - 1, r - 1);
+    if (tmp >= nodes) break;
+    n = tmp;
+    aspl += r * degree * std::pow(degree - 1, r - 1);
+    diam = r++;
+  }
+  diam++;
+  aspl += diam * (nodes - n);
+  aspl /= (nodes - 1);
+  *low_diam = diam;
+  *low_aspl = aspl;
+}
 
 }  // namespace gg
 #endif

@@ -67,6 +67,23 @@ def _grid_callback(evaluator):
     return L.EVAL_GRID_FN(_cb)
 
 
+def _sym_callback(evaluator):
+    if evaluator is None:
+        return ctypes.cast(None, L.EVAL_SYM_FN)
+
+    def _cb(ctx, n, d, b, g, c, adj_p, diam_p, sum_p, conn_p):
+        try:
+            adj = np.ctypeslib.as_array(adj_p, shape=(n, d)).copy()
+            ok, dm, S = evaluator(adj, b, g, c)
+            diam_p[0] = int(dm)
+            sum_p[0] = int(S)
+            conn_p[0] = int(bool(ok))
+            return 0
+        except Exception:  # surfaced as an error code
+            return -1
+    return L.EVAL_SYM_FN(_cb)
+
+
 def _result(rc, e, st, log, ld, ls) -> SaResult:
     if rc != L.GG_ALREADY_OPTIMAL:
         L.check(rc)
@@ -138,6 +155,50 @@ def sa_grid(edges, width: int, height: int, degree: int, length: int, seed: int,
     rc = L.lib().gg_sa_grid(ctypes.byref(shape), degree, L.ptr(e), L.ptr(rh), ctypes.byref(p),
                             fixed_temp, cb, None, ctypes.byref(st), L.ptr(ld),
                             ls.ctypes.data_as(L.u64p))
+    return _result(rc, e, st, log, ld, ls)
+
+
+GENERAL_FULL_FILE, GENERAL_BASE_FILE, GENERAL_BASE_CENTRES = 0, 1, 2
+
+
+def sa_general_sym(edges, nodes: int, degree: int, groups: int, added_centers: int, seed: int,
+                   ncalcs: int, max_temp: float = 100.0, min_temp: float = 0.217147,
+                   cooling_cycle: int = 1, hill_climbing: bool = False,
+                   evaluator: Optional[Callable] = None, log: int = 0) -> SaResult:
+    """The reference's sa() for rotationally symmetric graphs with centres
+    (general/sa.c:413-520).  `evaluator(adj, based_nodes, groups,
+    added_centers) -> (connected, diameter, S)` replaces the device evaluator."""
+    e = L.c32(edges).reshape(-1, 2).copy()
+    shape = L.GGGeneralShape(nodes, degree, groups, added_centers)
+    p = L.GGSaParams(ncalcs, max_temp, min_temp, cooling_cycle, int(hill_climbing), seed, log, 0)
+    st = L.GGSaStats()
+    ld = np.zeros(max(log, 1), dtype=np.int32)
+    ls = np.zeros(max(log, 1), dtype=np.uint64)
+    cb = _sym_callback(evaluator)
+    rc = L.lib().gg_sa_general_sym(ctypes.byref(shape), L.ptr(e), ctypes.byref(p), cb, None,
+                                   ctypes.byref(st), L.ptr(ld), ls.ctypes.data_as(L.u64p))
+    return _result(rc, e, st, log, ld, ls)
+
+
+def general_program_sym(file_edges, nodes: int, degree: int, groups: int, added_centers: int,
+                        source: int, seed: int, ncalcs: int, max_temp: float = 100.0,
+                        min_temp: float = 0.217147, hill_climbing: bool = False,
+                        evaluator: Optional[Callable] = None, log: int = 0) -> SaResult:
+    """The general program's run for any groups and centres (general/main.c:
+    360-462); file_edges are the lines read from the file."""
+    f = L.c32(file_edges).reshape(-1, 2)
+    lines = nodes * degree // 2
+    e = np.zeros((lines, 2), dtype=np.int32)
+    e[:len(f)] = f
+    shape = L.GGGeneralShape(nodes, degree, groups, added_centers)
+    p = L.GGSaParams(ncalcs, max_temp, min_temp, 1, int(hill_climbing), seed, log, 0)
+    st = L.GGSaStats()
+    ld = np.zeros(max(log, 1), dtype=np.int32)
+    ls = np.zeros(max(log, 1), dtype=np.uint64)
+    cb = _sym_callback(evaluator)
+    rc = L.lib().gg_general_program_sym(ctypes.byref(shape), L.ptr(e), source, len(f),
+                                        ctypes.byref(p), cb, None, ctypes.byref(st), L.ptr(ld),
+                                        ls.ctypes.data_as(L.u64p))
     return _result(rc, e, st, log, ld, ls)
 
 

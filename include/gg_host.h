@@ -101,6 +101,48 @@ int gg_general_program(int nodes, int degree, int32_t *edges, int halfway,
                        const gg_sa_params *params, gg_eval_fn eval, void *eval_ctx,
                        gg_sa_stats *stats, int32_t *log_diam, uint64_t *log_sum);
 
+/* ---- General SA with groups and added centres (general/sa.c:217-520,
+ * general/common.c:103-276): rotationally symmetric graphs (vertex v and
+ * v + based_nodes*k are the same vertex in group k; centres are the last
+ * added_centers ids), 2g moves and 1g moves as the reference draws them. */
+typedef struct gg_general_shape {
+  int32_t nodes;          /* all vertices, centres included */
+  int32_t degree;
+  int32_t groups;         /* -g */
+  int32_t added_centers;  /* -v */
+} gg_general_shape;
+
+/* Evaluator callback with the symmetry inputs (based_nodes =
+ * (nodes - added_centers) / groups). */
+typedef int (*gg_eval_sym_fn)(void *ctx, int nodes, int degree, int based_nodes, int groups,
+                              int added_centers, const int32_t *adj, int32_t *diameter,
+                              uint64_t *sum, int32_t *connected);
+
+/* sa() after srandom(params->seed); eval NULL = device evaluator. */
+int gg_sa_general_sym(const gg_general_shape *shape, int32_t *edges, const gg_sa_params *params,
+                      gg_eval_sym_fn eval, void *eval_ctx, gg_sa_stats *stats, int32_t *log_diam,
+                      uint64_t *log_sum);
+
+/* The general program's run (general/main.c:360-462) for any groups and
+ * centres.  edges: room for all nodes*degree/2 lines; source:
+ *   GG_GENERAL_FULL_FILE     -M: edges hold the whole graph;
+ *   GG_GENERAL_BASE_FILE     the first lines/groups lines are the base graph:
+ *                            rotated copies, 100 random moves, moves until
+ *                            connected (create_symmetric_edge);
+ *   GG_GENERAL_BASE_CENTRES  -v: the first file_lines lines are the file's;
+ *                            centre edges and the rest of the base lines are
+ *                            laid down, then the copies
+ *                            (create_symmetric_edge_with_vertexes);
+ * then the degree / self-loop checks, the start-graph check (groups = 1), the
+ * five timing moves' draws, and sa(). */
+#define GG_GENERAL_FULL_FILE 0
+#define GG_GENERAL_BASE_FILE 1
+#define GG_GENERAL_BASE_CENTRES 2
+int gg_general_program_sym(const gg_general_shape *shape, int32_t *edges, int source,
+                           int file_lines, const gg_sa_params *params, gg_eval_sym_fn eval,
+                           void *eval_ctx, gg_sa_stats *stats, int32_t *log_diam,
+                           uint64_t *log_sum);
+
 /* ---- Grid SA (grid/sa.c:283-361), same RNG draws (libc random() after
  * srandom(seed)) and moves as the reference: length-limited 2-opt with
  * groups-fold rotational symmetry (exchange_edge, grid/sa.c:137-221;

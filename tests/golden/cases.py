@@ -372,6 +372,16 @@ def program_cases():
         dict(id="gen_n200d5_wc", kind="general", input=("regular", 200, 5, 2),
              args="-s 11 -n 300 -w 50 -c 0.5"),
         dict(id="gen_n256d6", kind="general", input=("regular", 256, 6, 3), args="-s 2 -n 400"),
+        dict(id="gen_g2", kind="general", input=("regular", 32, 4, 11), args="-g 2 -s 3 -n 200"),
+        dict(id="gen_g3", kind="general", input=("regular", 20, 4, 12), args="-g 3 -s 4 -n 200"),
+        dict(id="gen_g4_H", kind="general", input=("regular", 24, 3, 13), args="-g 4 -H -s 5 -n 200"),
+        dict(id="gen_g2_v2", kind="general", input=("regular", 16, 3, 14),
+             args="-g 2 -v 2 -s 6 -n 200"),
+        dict(id="gen_g4_v4", kind="general", input=("regular", 24, 3, 15),
+             args="-g 4 -v 4 -s 7 -n 150 -w 20 -c 0.5"),
+        dict(id="gen_g2_M", kind="general", input=("output_of", "gen_g2"), args="-M -g 2 -s 8 -n 600"),
+        dict(id="gen_g2_v2_M", kind="general", input=("output_of", "gen_g2_v2"),
+             args="-M -g 2 -v 2 -s 9 -n 600 -w 5 -c 1"),
         dict(id="grid_lat_6x6", kind="grid", input=None, args="-W 6 -H 6 -D 4 -R 3 -s 5 -n 100"),
         dict(id="grid_lat_6x6_g2", kind="grid", input=None,
              args="-W 6 -H 6 -D 4 -R 2 -g 2 -s 5 -n 100"),
@@ -393,10 +403,15 @@ def program_cases():
     ]
 
 
-def write_program_input(case: dict, path: str) -> None:
-    """Writes a program case's start file (the reference's file formats)."""
+def write_program_input(case: dict, path: str, outputs=None) -> None:
+    """Writes a program case's start file (the reference's file formats).
+    ("output_of", id) copies the output file of an earlier case, taken from
+    `outputs` (id -> bytes)."""
     spec = case["input"]
-    if spec[0] == "regular":
+    if spec[0] == "output_of":
+        with open(path, "wb") as f:
+            f.write(outputs[spec[1]])
+    elif spec[0] == "regular":
         _, n, d, seed = spec
         G.write_edges(path, G.random_regular(n, d, seed))
     elif spec[0] == "grid":

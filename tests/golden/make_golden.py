@@ -81,13 +81,14 @@ def programs() -> None:
     import subprocess
     import tempfile
     rows = []
+    outputs = {}
     for case in program_cases():
         with tempfile.TemporaryDirectory() as tmp:
             inp, out = os.path.join(tmp, "in.txt"), os.path.join(tmp, "out.txt")
             args = []
             in_sha = ""
             if case["input"] is not None:
-                write_program_input(case, inp)
+                write_program_input(case, inp, outputs)
                 with open(inp, "rb") as f:
                     in_sha = hashlib.sha256(f.read()).hexdigest()
                 args = ["-f", inp]
@@ -98,7 +99,8 @@ def programs() -> None:
             if res.returncode != 0:
                 raise SystemExit(f"{case['id']}: {res.stdout[-500:]} {res.stderr[-500:]}")
             with open(out, "rb") as f:
-                out_sha = hashlib.sha256(f.read()).hexdigest()
+                outputs[case["id"]] = f.read()
+                out_sha = hashlib.sha256(outputs[case["id"]]).hexdigest()
             diam_line, steps = program_result_lines(res.stdout)
             rows.append(dict(case, input_sha256=in_sha, output_sha256=out_sha,
                              result=diam_line, steps=steps))

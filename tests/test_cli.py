@@ -23,8 +23,8 @@ with open(GOLDEN_PROGRAMS) as _f:
     GOLD = {r["id"]: r for r in json.load(_f)}
 
 
-def oracle_eval(adj):
-    r = oracle.evaluation(adj)
+def oracle_eval(adj, based_nodes=None, groups=1, added_centers=0):
+    r = oracle.evaluation(adj, based_nodes, groups, added_centers)
     return r.ok, r.diameter, r.S
 
 
@@ -33,14 +33,30 @@ def oracle_grid_eval(adj, groups, based_nodes, rotate_hash):
     return r.ok, r.diameter, r.S
 
 
+CASES = {c["id"]: c for c in program_cases()}
+
+
+def earlier_outputs(case, tmp_path, device):
+    """The output files a case's input is taken from, produced by running
+    the earlier cases through the CLI (each checked against its golden)."""
+    outs = {}
+    spec = case["input"]
+    if spec is not None and spec[0] == "output_of":
+        prev = CASES[spec[1]]
+        d = tmp_path / ("prev_" + prev["id"])
+        d.mkdir()
+        outs[prev["id"]] = run_program_case(prev, d, device)
+    return outs
+
+
 def run_program_case(case, tmp_path, device: bool, incremental=None):
     """Runs one golden program case through the CLI; checks it against the
-    reference program's recorded output."""
+    reference program's recorded output.  Returns the output file's bytes."""
     row = GOLD[case["id"]]
     args = []
     if case["input"] is not None:
         inp = str(tmp_path / "in.txt")
-        write_program_input(case, inp)
+        write_program_input(case, inp, earlier_outputs(case, tmp_path, device))
         with open(inp, "rb") as f:
             assert hashlib.sha256(f.read()).hexdigest() == row["input_sha256"]
         args = ["-f", inp]
@@ -53,10 +69,12 @@ def run_program_case(case, tmp_path, device: bool, incremental=None):
     else:
         cli.run_grid(args, evaluator=None if device else oracle_grid_eval, out=buf)
     with open(out, "rb") as f:
-        assert hashlib.sha256(f.read()).hexdigest() == row["output_sha256"]
+        data = f.read()
+    assert hashlib.sha256(data).hexdigest() == row["output_sha256"]
     diam_line, steps = program_result_lines(buf.getvalue())
     assert diam_line == row["result"]
     assert steps == row["steps"]
+    return data
 
 
 @pytest.mark.parametrize("case", program_cases(), ids=lambda c: c["id"])
@@ -88,7 +106,8 @@ def test_general_program_matches_live_reference(tmp_path):
         pytest.skip("reference build (oracle/_ref) not present")
     inp = str(tmp_path / "in.txt")
     G.write_edges(inp, G.random_regular(128, 5, 41))
-    for flags in ("-s 17 -n 500", "-M -s 4 -n 500 -w 10 -c 1"):
+    for flags in ("-s 17 -n 500", "-M -s 4 -n 500 -w 10 -c 1", "-g 2 -s 5 -n 300",
+                  "-g 4 -H -s 2 -n 300", "-g 3 -s 8 -n 300 -w 20 -c 2"):
         ref_out, my_out = str(tmp_path / "r.txt"), str(tmp_path / "m.txt")
         for p in (ref_out, my_out):
             if os.path.exists(p):
@@ -125,8 +144,9 @@ def test_grid_lattice_properties():
     (["-f", "x", "-s", "-1"], "-s value >= 0"),
     (["-f", "x", "-n", "0"], "-n value >= 1"),
     (["-f", "x", "-w", "0"], "-w value > 0"),
-    (["-f", "x", "-g", "2"], "groups = 1"),
-    (["-f", "x", "-v", "2"], "added centres"),
+    (["-f", "x", "-g", "2", "-D"], "When using -D option, -g must be 1."),
+    (["-f", "x", "-g", "0"], "-g value >= 1"),
+    (["-f", "x", "-v", "0"], "-v value > 0"),
     (["-f", "x", "-N"], "not supported"),
     (["-f", "/nonexistent/file"], "File not found"),
 ])
