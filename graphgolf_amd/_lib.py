@@ -44,7 +44,7 @@ class GGResult(ctypes.Structure):
 
 class GGGeneralShape(ctypes.Structure):
     _fields_ = [("nodes", ctypes.c_int32), ("degree", ctypes.c_int32), ("groups", ctypes.c_int32),
-                ("added_centers", ctypes.c_int32)]
+                ("added_centers", ctypes.c_int32), ("not_simple", ctypes.c_int32)]
 
 
 class GGSaParams(ctypes.Structure):
@@ -52,14 +52,15 @@ class GGSaParams(ctypes.Structure):
                 ("min_temp", ctypes.c_double), ("cooling_cycle", ctypes.c_int32),
                 ("hill_climbing", ctypes.c_int32), ("seed", ctypes.c_uint32),
                 ("log_capacity", ctypes.c_int32), ("incremental", ctypes.c_int32),
-                ("pad", ctypes.c_int32)]
+                ("detect_temp", ctypes.c_int32)]
 
 
 class GGSaStats(ctypes.Structure):
     _fields_ = [("steps", ctypes.c_int64), ("accepts", ctypes.c_int64),
                 ("evaluations", ctypes.c_int64), ("best_diameter", ctypes.c_int32),
                 ("pad", ctypes.c_int32), ("best_aspl", ctypes.c_double),
-                ("seconds_total", ctypes.c_double), ("seconds_eval", ctypes.c_double)]
+                ("seconds_total", ctypes.c_double), ("seconds_eval", ctypes.c_double),
+                ("max_diff_energy", ctypes.c_double)]
 
 
 EVAL_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, i32p,

@@ -16,8 +16,8 @@ start-graph moves (skipped with -M, which resumes from a whole graph written
 by -o), the same annealing moves, the same output file.  -B and -L choose the
 reference's evaluation algorithm; every algorithm gives the same results and
 this build always runs the device evaluator, so they only change the banner.
--D and -N stop with a message (temperature detection, non-simple graphs),
-as does the grid program's -d.  -O (listed by the
+-D / -d print the proposed temperatures as the reference does; -N lets
+repeated edges stay in the graph (the adjacency is rebuilt after each move).  -O (listed by the
 reference's help, absent from its option string) skips the graph checks.
 """
 from __future__ import annotations
@@ -118,6 +118,17 @@ def _outfile_free(path):
         raise CliError(f"Output file {path} exsits. ")
 
 
+def _detected(r, outfile, out):
+    """-D / -d: the proposed temperatures (general/main.c:464-470,
+    grid/main.c:749-755).  The reference has created the output file by then
+    and leaves it empty."""
+    print(f"Proposed max temperature is {(-1.0 * r.max_diff_energy) / math.log(0.5):f}", file=out)
+    print(f"Proposed min temperature is {(-2.0) / math.log(0.0001):f}", file=out)
+    if outfile:
+        open(outfile, "w").close()
+    return r
+
+
 def _results(r, low_diam, low_aspl, elapsed, sep, out):
     print("---", file=out)
     print(f"Diam. k = {r.best_diameter}  ASPL l = {r.best_aspl:f}  Diam. gap = "
@@ -155,8 +166,7 @@ def run_general(argv, evaluator=None, out=sys.stdout, incremental=None):
         raise CliError("Both -H and -D cannot be used.")
     if "D" in o and groups != 1:
         raise CliError("When using -D option, -g must be 1.")
-    if "D" in o or "N" in o:
-        raise CliError("-D and -N are not supported by this program")
+    detect, not_simple = "D" in o, "N" in o
     infile, outfile = o["f"], o.get("o")
     if not os.path.exists(infile):
         raise CliError("File not found")
@@ -190,7 +200,7 @@ def run_general(argv, evaluator=None, out=sys.stdout, incremental=None):
         source = SA.GENERAL_BASE_CENTRES
     elif halfway and centers:
         nodes += centers
-    symmetric = groups > 1 or centers > 0
+    symmetric = groups > 1 or centers > 0 or detect or not_simple
     if verify and not symmetric:
         verify_regular(nodes, degree, edges, out)
     low_diam, low_aspl = G.moore_bound(nodes, degree)
@@ -216,7 +226,8 @@ def run_general(argv, evaluator=None, out=sys.stdout, incremental=None):
     if symmetric:
         r = SA.general_program_sym(edges, nodes, degree, groups, centers, source, seed, ncalcs,
                                    max_temp=max_t, min_temp=min_t, hill_climbing=hill,
-                                   evaluator=evaluator)
+                                   evaluator=evaluator, not_simple=not_simple,
+                                   detect_temp=detect)
     else:
         if incremental is None:
             incremental = evaluator is None and nodes >= 4096 and degree <= 16
@@ -227,6 +238,8 @@ def run_general(argv, evaluator=None, out=sys.stdout, incremental=None):
     if r.already_optimal:
         print("The input file has already optimum solution.", file=out)
         return r
+    if detect:
+        return _detected(r, outfile, out)
     _results(r, low_diam, low_aspl, time.perf_counter() - t0, ",", out)
     if outfile:
         G.write_edges(outfile, r.edges)
@@ -269,12 +282,13 @@ def run_grid(argv, evaluator=None, out=sys.stdout):
         raise CliError('Must set -f or "-W and -H and -D"')
     if halfway and not infile:
         raise CliError("Must set both -M and -f")
-    if "d" in o:
-        raise CliError("-d is not supported by this program")
+    detect = "d" in o
     max_t = DEFAULT_MAX_TEMP if max_t is None else max_t
     min_t = DEFAULT_MIN_TEMP if min_t is None else min_t
     if max_t == min_t:
         raise CliError("The same values in -w and -c.")
+    if detect:
+        ncalcs = 100  # DEFAULT_DETECT_NCALS (grid/common.h:50)
     g = groups
     if infile:  # grid/main.c:631-668
         if not os.path.exists(infile):
@@ -336,10 +350,13 @@ def run_grid(argv, evaluator=None, out=sys.stdout):
     t0 = time.perf_counter()
     r = SA.grid_program(e, width, height, degree, length, seed, ncalcs, source, groups=g,
                         max_temp=max_t, min_temp=min_t, cooling_cycle=cycle, hill_climbing=hill,
-                        fixed_temp=0.0 if hill or fixed is None else fixed, evaluator=evaluator)
+                        fixed_temp=0.0 if hill or fixed is None else fixed, evaluator=evaluator,
+                        detect_temp=detect)
     if r.already_optimal:
         print("The input file has already optimum solution.", file=out)
         return r
+    if detect:
+        return _detected(r, outfile, out)
     _results(r, low_diam, low_aspl, time.perf_counter() - t0, " ", out)
     if outfile:
         G.write_grid_edges(outfile, r.edges, height)

@@ -980,6 +980,7 @@ int sa_grid_run(const gg_grid_shape *shape, int degree, int32_t *edges, const in
   best_diam = cur_diam;
   best_aspl = cur_aspl;
   int64_t accepts = 0, ii;
+  double max_diff = -1;  // NOT_N_DEFINED (grid/main.c:597)
   for (ii = 0; ii < P->ncalcs; ++ii) {
     int nd = 0;
     double na = 0;
@@ -993,7 +994,10 @@ int sa_grid_run(const gg_grid_shape *shape, int degree, int32_t *edges, const in
     // accept (grid/sa.c:223-268)
     const double diff = ((cur_aspl - na) * nodes * (nodes - 1)) / groups;
     bool acc;
-    if (nd < cur_diam) acc = true;
+    if (P->detect_temp) {  // every proposal faces the Metropolis test (grid/sa.c:231-233)
+      max_diff = std::max(max_diff, -1.0 * diff);
+      acc = std::exp(diff / (fixed ? fixed_temp : temp)) > ref_uniform();
+    } else if (nd < cur_diam) acc = true;
     else if (nd > cur_diam) acc = false;
     else if (diff >= 0) acc = true;
     else if (P->hill_climbing) acc = false;
@@ -1021,6 +1025,7 @@ int sa_grid_run(const gg_grid_shape *shape, int degree, int32_t *edges, const in
     stats->best_aspl = best_aspl;
     stats->seconds_total = now_s() - t_start;
     stats->seconds_eval = t_eval;
+    stats->max_diff_energy = max_diff;
   }
   return GG_OK;
 }

@@ -44,6 +44,7 @@ bool same_edge(int a0, int a1, int b0, int b1) { return (a0 == b0 && a1 == b1) |
 
 struct SymState {
   int n, d, g, c, lines, bl, bn;
+  bool simple = true;  // false (-N): moves leave the adjacency to be rebuilt
   int32_t *edges;
   std::vector<int32_t> adj;
   // What the last move changed (restore_adj / restore_edge).
@@ -148,6 +149,15 @@ struct SymState {
     if (!fresh(tl.data(), g, (const int(*)[2])te.data(), 1, pattern == g)) return GG_OK;
     for (int i = 0; i < g; ++i)
       if (order(te[2 * i], te[2 * i + 1]) == kRight) std::swap(te[2 * i], te[2 * i + 1]);
+    if (!simple) {
+      for (int i = 0; i < g; ++i) {
+        edges[2 * tl[i]] = te[2 * i];
+        edges[2 * tl[i] + 1] = te[2 * i + 1];
+      }
+      kind = 1;
+      *done = true;
+      return GG_OK;
+    }
     // The row slot of every endpoint's old edge (the hash of the reference).
     std::vector<int> slot(n, -1);
     for (int i = 0; i < g; ++i) {
@@ -237,6 +247,15 @@ struct SymState {
     }
     for (int i = 0; i < 2 * g; ++i)
       if (order(te[2 * i], te[2 * i + 1]) == kRight) std::swap(te[2 * i], te[2 * i + 1]);
+    if (!simple) {
+      for (int i = 0; i < g; ++i)
+        for (int k = 0; k < 2; ++k) {
+          edges[2 * tl[2 * i + k]] = te[2 * (2 * i + k)];
+          edges[2 * tl[2 * i + k] + 1] = te[2 * (2 * i + k) + 1];
+        }
+      kind = 2;
+      return GG_OK;
+    }
     // Slots and restore values from the unchanged adjacency, then the changes.
     std::vector<int> ys(4 * g), xs(4 * g);
     for (int i = 0; i < g; ++i) {
@@ -317,10 +336,13 @@ int sa_sym_run(const gg_general_shape *sh, int32_t *edges, const gg_sa_params *P
   const double t_start = now_s();
   double t_eval = 0;
   SymState S(nodes, degree, groups, cen, edges);
+  S.simple = !sh->not_simple;
   int rc = S.init();
   if (rc) return rc;
   const int lines = S.lines;
   std::vector<int32_t> best(edges, edges + 2 * (size_t)lines);
+  std::vector<int32_t> saved;  // -N: the edges before the current move
+  double max_diff = 0;         // general/main.c:327
   int64_t logn = 0, evals = 0;
   auto evaluate = [&](int *diam, double *aspl) -> int {
     int32_t dm = 0, conn = 0;
@@ -363,12 +385,18 @@ int sa_sym_run(const gg_general_shape *sh, int32_t *edges, const gg_sa_params *P
     int nd;
     double na;
     while (true) {
-      if (restore) S.restore();
+      if (S.simple) {
+        if (restore) S.restore();
+      } else {
+        saved.assign(edges, edges + 2 * (size_t)lines);
+      }
       if ((rc = S.exchange())) return rc;
+      if (!S.simple && (rc = S.init())) return rc;  // create_adj (general/sa.c:463-464)
       rc = evaluate(&nd, &na);
       if (rc < 0) return rc;
       if (rc == 1) break;
-      restore = true;
+      if (S.simple) restore = true;
+      else std::copy(saved.begin(), saved.end(), edges);
     }
     bool acc;  // accept (general/sa.c:371-411)
     if (nd < cur_diam) acc = true;
@@ -377,10 +405,12 @@ int sa_sym_run(const gg_general_shape *sh, int32_t *edges, const gg_sa_params *P
     else if (P->hill_climbing) acc = false;
     else {
       const double diff = ((cur_aspl - na) * nodes * (nodes - 1)) / groups;
+      if (P->detect_temp) max_diff = std::max(max_diff, -1.0 * diff);
       acc = std::exp(diff / temp) > ref_uniform();
     }
     if (!acc) {
-      restore = true;
+      if (S.simple) restore = true;
+      else std::copy(saved.begin(), saved.end(), edges);
     } else {
       restore = false;
       ++accepts;
@@ -404,6 +434,7 @@ int sa_sym_run(const gg_general_shape *sh, int32_t *edges, const gg_sa_params *P
     stats->best_aspl = best_aspl;
     stats->seconds_total = now_s() - t_start;
     stats->seconds_eval = t_eval;
+    stats->max_diff_energy = max_diff;
   }
   return GG_OK;
 }
@@ -474,11 +505,13 @@ extern "C" int gg_general_program_sym(const gg_general_shape *shape, int32_t *ed
         edges[2 * (bl * j + i) + 1] = edges[2 * i + 1] + based * j;
       }
     SymState S(nodes, degree, groups, 0, edges);
+    S.simple = !shape->not_simple;
     if ((rc = S.init())) return rc;
     for (int i = 0; i < 100; ++i)
       if ((rc = S.exchange())) return rc;
     while (true) {
       if ((rc = S.exchange())) return rc;
+      if (!S.simple && (rc = S.init())) return rc;
       int32_t dm = 0, conn = 0;
       uint64_t sum = 0;
       if ((rc = ev(ctx, nodes, degree, based, groups, 0, S.adj.data(), &dm, &sum, &conn))) return rc;
@@ -522,9 +555,12 @@ extern "C" int gg_general_program_sym(const gg_general_shape *shape, int32_t *ed
   {  // estimate_elapse_time's five moves on a copy (general/sa.c:523-554)
     std::vector<int32_t> copy(edges, edges + 2 * (size_t)lines);
     SymState T(nodes, degree, groups, cen, copy.data());
+    T.simple = !shape->not_simple;
     if ((rc = T.init())) return rc;
-    for (int i = 0; i < 5; ++i)
+    for (int i = 0; i < 5; ++i) {
       if ((rc = T.exchange())) return rc;
+      if (!T.simple && (rc = T.init())) return rc;
+    }
   }
   return sa_sym_run(shape, edges, P, eval, ctx, stats, log_diam, log_sum, false);
 }

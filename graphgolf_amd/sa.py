@@ -29,6 +29,7 @@ class SaResult:
     log_diameter: Optional[np.ndarray] = None
     log_sum: Optional[np.ndarray] = None
     already_optimal: bool = False
+    max_diff_energy: float = 0.0  # with detect_temp: the largest energy increase tested
 
 
 def _general_callback(evaluator):
@@ -91,7 +92,8 @@ def _result(rc, e, st, log, ld, ls) -> SaResult:
     return SaResult(e, st.steps, st.accepts, st.evaluations, st.best_diameter, st.best_aspl,
                     st.seconds_total, st.seconds_eval,
                     ld[:n_log] if log else None, ls[:n_log] if log else None,
-                    already_optimal=rc == L.GG_ALREADY_OPTIMAL)
+                    already_optimal=rc == L.GG_ALREADY_OPTIMAL,
+                    max_diff_energy=float(st.max_diff_energy))
 
 
 def sa_general(edges, nodes: int, degree: int, seed: int, ncalcs: int, max_temp: float = 100.0,
@@ -183,15 +185,17 @@ def sa_general_sym(edges, nodes: int, degree: int, groups: int, added_centers: i
 def general_program_sym(file_edges, nodes: int, degree: int, groups: int, added_centers: int,
                         source: int, seed: int, ncalcs: int, max_temp: float = 100.0,
                         min_temp: float = 0.217147, hill_climbing: bool = False,
-                        evaluator: Optional[Callable] = None, log: int = 0) -> SaResult:
+                        evaluator: Optional[Callable] = None, log: int = 0,
+                        not_simple: bool = False, detect_temp: bool = False) -> SaResult:
     """The general program's run for any groups and centres (general/main.c:
     360-462); file_edges are the lines read from the file."""
     f = L.c32(file_edges).reshape(-1, 2)
     lines = nodes * degree // 2
     e = np.zeros((lines, 2), dtype=np.int32)
     e[:len(f)] = f
-    shape = L.GGGeneralShape(nodes, degree, groups, added_centers)
-    p = L.GGSaParams(ncalcs, max_temp, min_temp, 1, int(hill_climbing), seed, log, 0)
+    shape = L.GGGeneralShape(nodes, degree, groups, added_centers, int(not_simple))
+    p = L.GGSaParams(ncalcs, max_temp, min_temp, 1, int(hill_climbing), seed, log, 0,
+                     int(detect_temp))
     st = L.GGSaStats()
     ld = np.zeros(max(log, 1), dtype=np.int32)
     ls = np.zeros(max(log, 1), dtype=np.uint64)
@@ -208,7 +212,8 @@ GRID_LATTICE, GRID_BASE_FILE, GRID_FULL_FILE = 0, 1, 2
 def grid_program(edges, width: int, height: int, degree: int, length: int, seed: int, ncalcs: int,
                  source: int, groups: int = 1, max_temp: float = 100.0, min_temp: float = 0.217147,
                  cooling_cycle: int = 1, hill_climbing: bool = False, fixed_temp: float = 0.0,
-                 evaluator: Optional[Callable] = None, log: int = 0) -> SaResult:
+                 evaluator: Optional[Callable] = None, log: int = 0,
+                 detect_temp: bool = False) -> SaResult:
     """The grid program's run (grid/main.c:630-727): the lattice (source
     GRID_LATTICE, `edges` ignored), the symmetric copies (GRID_BASE_FILE with
     groups > 1: `edges` holds the base grid's lines in base coordinates), the
@@ -223,7 +228,8 @@ def grid_program(edges, width: int, height: int, degree: int, length: int, seed:
         e[:len(src)] = src
     rh = make_rotate_hash(nodes, height, width, groups)
     shape = L.GGGridShape(width, height, groups, length)
-    p = L.GGSaParams(ncalcs, max_temp, min_temp, cooling_cycle, int(hill_climbing), seed, log)
+    p = L.GGSaParams(ncalcs, max_temp, min_temp, cooling_cycle, int(hill_climbing), seed, log, 0,
+                     int(detect_temp))
     st = L.GGSaStats()
     ld = np.zeros(max(log, 1), dtype=np.int32)
     ls = np.zeros(max(log, 1), dtype=np.uint64)

@@ -64,7 +64,9 @@ typedef struct gg_sa_params {
   int32_t incremental;   /* general SA on the device evaluator: score each proposal
                             with the incremental scorer (include/gg_incr.h; simple graphs,
                             degree <= 16).  Same exact results, so the same trajectory. */
-  int32_t pad;
+  int32_t detect_temp;   /* -D / -d: record the largest energy increase met
+                            (stats->max_diff_energy); the grid SA then accepts every
+                            proposal by the Metropolis test alone (grid/sa.c:231-233) */
 } gg_sa_params;
 
 typedef struct gg_sa_stats {
@@ -76,6 +78,7 @@ typedef struct gg_sa_stats {
   double best_aspl;
   double seconds_total;
   double seconds_eval;
+  double max_diff_energy; /* with detect_temp: max over tested proposals of -diff */
 } gg_sa_stats;
 
 int gg_sa_general(int nodes, int degree, int32_t *edges, const gg_sa_params *params,
@@ -110,6 +113,8 @@ typedef struct gg_general_shape {
   int32_t degree;
   int32_t groups;         /* -g */
   int32_t added_centers;  /* -v */
+  int32_t not_simple;     /* -N: repeated edges allowed; the adjacency is rebuilt
+                             after every move and moves are undone from a copy */
 } gg_general_shape;
 
 /* Evaluator callback with the symmetry inputs (based_nodes =

@@ -382,6 +382,11 @@ def program_cases():
         dict(id="gen_g2_M", kind="general", input=("output_of", "gen_g2"), args="-M -g 2 -s 8 -n 600"),
         dict(id="gen_g2_v2_M", kind="general", input=("output_of", "gen_g2_v2"),
              args="-M -g 2 -v 2 -s 9 -n 600 -w 5 -c 1"),
+        dict(id="gen_n64d4_D", kind="general", input=("regular", 64, 4, 7), args="-D -s 3 -n 300"),
+        dict(id="gen_multi_N", kind="general", input=("multi", 60, 4, 21), args="-N -s 4 -n 300"),
+        dict(id="gen_multi_N_g2", kind="general", input=("multi", 40, 4, 22),
+             args="-N -g 2 -s 5 -n 300"),
+        dict(id="grid_lat_8x8_d", kind="grid", input=None, args="-W 8 -H 8 -D 4 -R 3 -s 4 -d"),
         dict(id="grid_lat_6x6", kind="grid", input=None, args="-W 6 -H 6 -D 4 -R 3 -s 5 -n 100"),
         dict(id="grid_lat_6x6_g2", kind="grid", input=None,
              args="-W 6 -H 6 -D 4 -R 2 -g 2 -s 5 -n 100"),
@@ -414,6 +419,20 @@ def write_program_input(case: dict, path: str, outputs=None) -> None:
     elif spec[0] == "regular":
         _, n, d, seed = spec
         G.write_edges(path, G.random_regular(n, d, seed))
+    elif spec[0] == "multi":  # a regular graph with one repeated edge
+        _, n, d, seed = spec
+        e = G.random_regular(n, d, seed).copy()
+        keys = set(map(tuple, np.sort(e, axis=1).tolist()))
+        rng = np.random.default_rng(seed)
+        while True:
+            t0, t1 = rng.choice(len(e), 2, replace=False)
+            a, b = e[t0]
+            c, dd = e[t1]
+            if len({a, b, c, dd}) == 4 and (min(a, c), max(a, c)) in keys:
+                e[t0] = (a, c)
+                e[t1] = (b, dd)
+                break
+        G.write_edges(path, e)
     elif spec[0] == "grid":
         _, w, h, d, r, seed = spec
         G.write_grid_edges(path, G.random_grid(w, h, d, r, seed), h)
@@ -423,8 +442,10 @@ def write_program_input(case: dict, path: str, outputs=None) -> None:
 
 
 def program_result_lines(text: str):
-    """The "Diam. k = ..." line and the step count of a program's output."""
+    """The "Diam. k = ..." line (or, for -D / -d, the "Proposed ..." lines)
+    and the step count of a program's output."""
     diam = [ln for ln in text.splitlines() if ln.startswith("Diam. k = ")]
+    prop = [ln for ln in text.splitlines() if ln.startswith("Proposed ")]
     steps = [ln for ln in text.splitlines() if ln.startswith("Steps: ")]
     n = int(steps[-1].split()[1].rstrip(",")) if steps else -1
-    return (diam[-1] if diam else ""), n
+    return (diam[-1] if diam else " | ".join(prop)), n

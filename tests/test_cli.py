@@ -147,7 +147,6 @@ def test_grid_lattice_properties():
     (["-f", "x", "-g", "2", "-D"], "When using -D option, -g must be 1."),
     (["-f", "x", "-g", "0"], "-g value >= 1"),
     (["-f", "x", "-v", "0"], "-v value > 0"),
-    (["-f", "x", "-N"], "not supported"),
     (["-f", "/nonexistent/file"], "File not found"),
 ])
 def test_general_argument_errors(argv, msg):
@@ -165,7 +164,7 @@ def test_general_argument_errors(argv, msg):
     (["-W", "4", "-H", "4", "-D", "3", "-R", "2", "-g", "3"], "-g value == 1 or 2 or 4"),
     (["-W", "6", "-H", "4", "-D", "4", "-R", "2", "-g", "4"], "When g = 4, width"),
     (["-W", "4", "-H", "5", "-D", "4", "-R", "2", "-g", "2"], "When g = 2, height"),
-    (["-W", "4", "-H", "4", "-D", "3", "-R", "2", "-d"], "not supported"),
+    (["-W", "4", "-H", "4", "-D", "3", "-R", "2", "-d", "-Y"], "Both -Y and -d"),
 ])
 def test_grid_argument_errors(argv, msg):
     with pytest.raises(cli.CliError, match=msg.replace("(", r"\(").replace(")", r"\)")):
