@@ -220,14 +220,73 @@ gg_bsa_propose(int count, uint64_t seed, uint64_t step, int lines, const int32_t
   l[2] = valid ? 1 : 0;
 }
 
-// Writes changed lines (line, u, v) into the device edge list.
+// Grid graphs (vertex v at column v / height, row v % height, every edge at
+// most `length` Manhattan steps): line t0 as above, then a vertex c within
+// the length of t0's first endpoint (offsets: packed (dx, dy) of the length
+// ball) and t1 one of c's lines (lineat[c*d + j]), as the host's local draw
+// does, so that most draws respect the limit; valid also needs both added
+// edges within the length (grid/sa.c:196-201).
 __global__ void __launch_bounds__(256)
-gg_bsa_set_lines(const int32_t *__restrict__ upd, int count, int32_t *__restrict__ edges) {
+gg_bsa_propose_grid(int count, uint64_t seed, uint64_t step, int lines, const int32_t *__restrict__ edges,
+                    const int32_t *__restrict__ adj, const int32_t *__restrict__ lineat, int d, int n,
+                    int height, int length, const int32_t *__restrict__ offsets, int noff,
+                    int32_t *__restrict__ out, int32_t *__restrict__ lines_out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const uint64_t base = bsa_mix(seed ^ bsa_mix(step * 0x100000001B3ull + (uint64_t)i));
+  const int width = n / height;
+  auto near = [&](int u, int v) {
+    const int dx = u / height - v / height, dy = u % height - v % height;
+    return (dx < 0 ? -dx : dx) + (dy < 0 ? -dy : dy) <= length;
+  };
+  int t0 = 0, t1 = 0, r = 0, a = 0, b = 0, c = 0, e = 0;
+  bool ok = false;
+  for (int k = 0; k < 64 && !ok; ++k) {
+    const uint64_t x = bsa_mix(base + (uint64_t)k), y = bsa_mix(x);
+    t0 = (int)(((x & 0xffffffffull) * (uint64_t)lines) >> 32);
+    a = edges[2 * t0];
+    b = edges[2 * t0 + 1];
+    const int o = offsets[(int)(((x >> 32) * (uint64_t)noff) >> 32)];
+    const int cx = a / height + (o / 4096 - 1024), cy = a % height + (o % 4096 - 1024);
+    if (cx < 0 || cy < 0 || cy >= height || cx >= width) continue;
+    const int cv = cx * height + cy;
+    t1 = lineat[(size_t)cv * d + (int)(((y & 0xffffffffull) * (uint64_t)d) >> 32)];
+    r = (int)((y >> 63) & 1u);
+    c = edges[2 * t1];
+    e = edges[2 * t1 + 1];
+    ok = t0 != t1 && a != c && a != e && b != c && b != e;
+  }
+  bool valid = ok;
+  if (ok) {
+    const int q1 = r ? c : e, p2 = r ? b : c, q2 = r ? e : b;
+    valid = near(a, q1) && near(p2, q2) && !row_has(adj, d, a, q1) && !row_has(adj, d, p2, q2);
+  }
+  int32_t *o = out + 5 * (size_t)i;
+  o[0] = a;
+  o[1] = b;
+  o[2] = c;
+  o[3] = e;
+  o[4] = r;
+  int32_t *l = lines_out + 3 * (size_t)i;
+  l[0] = t0;
+  l[1] = t1;
+  l[2] = valid ? 1 : 0;
+}
+
+// Writes changed lines (line, u, v) into the device edge list, and records
+// (-1 - slot, line, 0) into the grid line table.
+__global__ void __launch_bounds__(256)
+gg_bsa_set_lines(const int32_t *__restrict__ upd, int count, int32_t *__restrict__ edges,
+                 int32_t *__restrict__ lineat) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
   const int t = upd[3 * i];
-  edges[2 * t] = upd[3 * i + 1];
-  edges[2 * t + 1] = upd[3 * i + 2];
+  if (t >= 0) {
+    edges[2 * t] = upd[3 * i + 1];
+    edges[2 * t + 1] = upd[3 * i + 2];
+  } else if (lineat) {
+    lineat[-1 - t] = upd[3 * i + 1];
+  }
 }
 
 struct Prop {
@@ -488,22 +547,40 @@ extern "C" int gg_apply_move(int lines, int32_t *edges, int line0, int line1, in
 // The device proposal draw of ranked steps, for tests: `count` proposals of
 // the graph (edges [lines][2], adjacency [n][d]) for (seed, step).
 extern "C" int gg_sa_proposals(int nodes, int degree, const int32_t *edges, const int32_t *adj,
-                               int count, uint64_t seed, uint64_t step, int32_t *props,
-                               int32_t *lines_out) {
+                               int count, uint64_t seed, uint64_t step, int grid_height,
+                               int grid_length, int32_t *props, int32_t *lines_out) {
   if (!edges || !adj || !props || !lines_out || nodes < 4 || degree < 1 || count < 1)
     return set_error(GG_ERR_INVALID, "bad proposal arguments");
+  if (grid_height > 0 && (nodes % grid_height || grid_length < 1))
+    return set_error(GG_ERR_INVALID, "bad grid shape");
   int rc = gg_init(-1);
   if (rc) return rc;
   const int lines = nodes * degree / 2;
-  DevArr<int32_t> e_d, a_d, p_d, l_d;
+  DevArr<int32_t> e_d, a_d, p_d, l_d, la_d, of_d;
   if ((rc = e_d.alloc(2 * (size_t)lines)) || (rc = a_d.alloc((size_t)nodes * degree)) ||
       (rc = p_d.alloc((size_t)count * 5)) || (rc = l_d.alloc((size_t)count * 3)))
     return rc;
   hipStream_t es = gg::engine_stream();
   BSA_HIP(hipMemcpyAsync(e_d.p, edges, 2 * (size_t)lines * 4, hipMemcpyHostToDevice, es));
   BSA_HIP(hipMemcpyAsync(a_d.p, adj, (size_t)nodes * degree * 4, hipMemcpyHostToDevice, es));
-  hipLaunchKernelGGL(gg_bsa_propose, dim3((count + 255) / 256), dim3(256), 0, es, count, seed, step,
-                     lines, (const int32_t *)e_d.p, (const int32_t *)a_d.p, degree, p_d.p, l_d.p);
+  if (grid_height > 0) {
+    std::vector<int32_t> e(edges, edges + 2 * (size_t)lines);
+    SaGraph g(nodes, degree, e.data());
+    if ((rc = g.init())) return rc;
+    g.grid_height = grid_height;
+    g.grid_length = grid_length;
+    g.grid_setup();
+    if ((rc = la_d.alloc(g.lineat.size())) || (rc = of_d.alloc(g.offsets.size()))) return rc;
+    BSA_HIP(hipMemcpy(la_d.p, g.lineat.data(), g.lineat.size() * 4, hipMemcpyHostToDevice));
+    BSA_HIP(hipMemcpy(of_d.p, g.offsets.data(), g.offsets.size() * 4, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(gg_bsa_propose_grid, dim3((count + 255) / 256), dim3(256), 0, es, count, seed,
+                       step, lines, (const int32_t *)e_d.p, (const int32_t *)a_d.p,
+                       (const int32_t *)la_d.p, degree, nodes, grid_height, grid_length,
+                       (const int32_t *)of_d.p, (int)g.offsets.size(), p_d.p, l_d.p);
+  } else {
+    hipLaunchKernelGGL(gg_bsa_propose, dim3((count + 255) / 256), dim3(256), 0, es, count, seed, step,
+                       lines, (const int32_t *)e_d.p, (const int32_t *)a_d.p, degree, p_d.p, l_d.p);
+  }
   BSA_HIP(hipGetLastError());
   BSA_HIP(hipMemcpyAsync(props, p_d.p, (size_t)count * 5 * 4, hipMemcpyDeviceToHost, es));
   BSA_HIP(hipMemcpyAsync(lines_out, l_d.p, (size_t)count * 3 * 4, hipMemcpyDeviceToHost, es));
@@ -626,7 +703,7 @@ extern "C" int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa
   vs.maxd = std::min(std::max(P->hist_maxd, 2), 15);
   const int cap = 16384;
   const int bt = std::max(P->batch_train, 32);
-  DevArr<int32_t> props_d, idx_d, tidx_d, edges_d, plines_d, upd_d;
+  DevArr<int32_t> props_d, idx_d, tidx_d, edges_d, plines_d, upd_d, lineat_d, offsets_d;
   struct Pinned {
     int32_t *props = nullptr, *lines = nullptr, *upd = nullptr;
     ~Pinned() {
@@ -649,14 +726,23 @@ extern "C" int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa
         (rc = vstat_d.alloc((size_t)nodes * kVStat)) ||
         (rc = idx_d.alloc(2 * (size_t)std::max(npmax, bt))) || (rc = tidx_d.alloc((size_t)bt)) ||
         (rc = edges_d.alloc(2 * (size_t)g.lines)) || (rc = plines_d.alloc((size_t)npmax * 3)) ||
-        (rc = upd_d.alloc(3 * 2 * (size_t)kMaxCombine)))
+        (rc = upd_d.alloc(3 * 6 * (size_t)kMaxCombine)))
       return rc;
     if (hipHostMalloc((void **)&pin.props, (size_t)npmax * 5 * 4) != hipSuccess ||
         hipHostMalloc((void **)&pin.lines, (size_t)npmax * 3 * 4) != hipSuccess ||
-        hipHostMalloc((void **)&pin.upd, 3 * 2 * (size_t)kMaxCombine * 4) != hipSuccess)
+        hipHostMalloc((void **)&pin.upd, 3 * 6 * (size_t)kMaxCombine * 4) != hipSuccess)
       return set_error(GG_ERR_HIP, "pinned allocation failed");
     if (hipMemcpy(edges_d.p, edges, 2 * (size_t)g.lines * 4, hipMemcpyHostToDevice) != hipSuccess)
       return set_error(GG_ERR_HIP, "edge list upload failed");
+    if (g.grid_height > 0) {  // the line table and length ball of the grid draws
+      if ((rc = lineat_d.alloc(g.lineat.size())) || (rc = offsets_d.alloc(g.offsets.size())))
+        return rc;
+      if (hipMemcpy(lineat_d.p, g.lineat.data(), g.lineat.size() * 4, hipMemcpyHostToDevice) !=
+              hipSuccess ||
+          hipMemcpy(offsets_d.p, g.offsets.data(), g.offsets.size() * 4, hipMemcpyHostToDevice) !=
+              hipSuccess)
+        return set_error(GG_ERR_HIP, "grid table upload failed");
+    }
     const double tf = now_s();
     rc = vs.refresh(g);
     if (!rc && hipMemcpy(vstat_d.p, vs.host.data(), vs.host.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
@@ -688,8 +774,8 @@ extern "C" int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa
     // --- proposals
     double tp = now_s();
     const int np = explore ? E : NP;
-    // Ranked steps of general graphs draw their proposals on the device.
-    const bool dev_props = use_sur && !explore && g.grid_height == 0;
+    // Ranked steps draw their proposals on the device.
+    const bool dev_props = use_sur && !explore;
     for (int i = 0; i < (dev_props ? 0 : np); ++i) {
       Prop &p = props[i];
       if (explore) {
@@ -715,7 +801,14 @@ extern "C" int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa
     // --- features + ranking on the device
     if (use_sur) {
       const double ts = now_s();
-      if (dev_props) {
+      if (dev_props && g.grid_height > 0) {
+        hipLaunchKernelGGL(gg_bsa_propose_grid, dim3((np + 255) / 256), dim3(256), 0, es, np,
+                           P->seed, (uint64_t)step, g.lines, (const int32_t *)edges_d.p,
+                           (const int32_t *)gg_plan_base_device(plan.p), (const int32_t *)lineat_d.p,
+                           degree, nodes, g.grid_height, g.grid_length, (const int32_t *)offsets_d.p,
+                           (int)g.offsets.size(), props_d.p, plines_d.p);
+        BSA_HIP(hipGetLastError());
+      } else if (dev_props) {
         hipLaunchKernelGGL(gg_bsa_propose, dim3((np + 255) / 256), dim3(256), 0, es, np, P->seed,
                            (uint64_t)step, g.lines, (const int32_t *)edges_d.p,
                            (const int32_t *)gg_plan_base_device(plan.p), degree, props_d.p,
@@ -935,9 +1028,21 @@ extern "C" int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa
               ++nu;
             }
           }
-          BSA_HIP(hipMemcpyAsync(upd_d.p, pin.upd, (size_t)nu * 3 * 4, hipMemcpyHostToDevice, es));
-          hipLaunchKernelGGL(gg_bsa_set_lines, dim3((nu + 255) / 256), dim3(256), 0, es,
-                             (const int32_t *)upd_d.p, nu, edges_d.p);
+          // Grid graphs: the line table entries the moves rewrote, as
+          // (-1 - slot, line, 0) records after the lines.
+          int nl = 0;
+          if (g.grid_height > 0)
+            for (int k = 0; k < C.nm; ++k)
+              for (int j = 0; j < 4; ++j) {
+                const int at = C.sl[k].at[j];
+                pin.upd[3 * (nu + nl)] = -1 - at;
+                pin.upd[3 * (nu + nl) + 1] = g.lineat[at];
+                pin.upd[3 * (nu + nl) + 2] = 0;
+                ++nl;
+              }
+          BSA_HIP(hipMemcpyAsync(upd_d.p, pin.upd, (size_t)(nu + nl) * 3 * 4, hipMemcpyHostToDevice, es));
+          hipLaunchKernelGGL(gg_bsa_set_lines, dim3((nu + nl + 255) / 256), dim3(256), 0, es,
+                             (const int32_t *)upd_d.p, nu + nl, edges_d.p, lineat_d.p);
           BSA_HIP(hipGetLastError());
         }
         if (incr.p) {

@@ -352,12 +352,14 @@ def sa_features(adj: np.ndarray, props: np.ndarray, vstat: np.ndarray | None = N
     return out
 
 
-def sa_proposals(edges, nodes: int, degree: int, count: int, seed: int, step: int):
+def sa_proposals(edges, nodes: int, degree: int, count: int, seed: int, step: int,
+                 grid_height: int = 0, grid_length: int = 0):
     """The device move draw of ranked batched-SA steps: (props [count, 5] =
     (a, b, c, d, r), lines [count, 3] = (t0, t1, valid))."""
     lib = L.lib()
     lib.gg_sa_proposals.argtypes = [ctypes.c_int, ctypes.c_int, L.i32p, L.i32p, ctypes.c_int,
-                                    ctypes.c_uint64, ctypes.c_uint64, L.i32p, L.i32p]
+                                    ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, ctypes.c_int,
+                                    L.i32p, L.i32p]
     lib.gg_sa_proposals.restype = ctypes.c_int
     from .graphs import edges_to_adj
     e = L.c32(edges).reshape(-1, 2)
@@ -365,5 +367,5 @@ def sa_proposals(edges, nodes: int, degree: int, count: int, seed: int, step: in
     props = np.zeros((count, 5), dtype=np.int32)
     lines = np.zeros((count, 3), dtype=np.int32)
     L.check(lib.gg_sa_proposals(nodes, degree, L.ptr(e), L.ptr(adj), count, seed, step,
-                                L.ptr(props), L.ptr(lines)))
+                                grid_height, grid_length, L.ptr(props), L.ptr(lines)))
     return props, lines

@@ -295,14 +295,16 @@ int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa_params *pa
 int gg_sa_features(int nodes, int degree, const int32_t *adj, int count, const int32_t *props,
                    const float *vstat, float *feats);
 
-/* Ranked steps of general graphs draw their proposals on the device: lines
- * t0 != t1 and a variant r from a counter-based stream of (seed, step,
- * proposal), redrawn (up to 64 times) while the lines share a vertex, valid
- * when neither added edge exists.  props[i] = (a, b, c, d, r) with (a, b) =
- * line t0 and (c, d) = line t1; lines_out[i] = (t0, t1, valid).  Exposed for
- * tests. */
+/* Ranked steps draw their proposals on the device: lines t0 != t1 and a
+ * variant r from a counter-based stream of (seed, step, proposal), redrawn
+ * (up to 64 times) while the lines share a vertex, valid when neither added
+ * edge exists.  Grid graphs (grid_height > 0) draw t1 among the lines of a
+ * vertex within grid_length of t0's first endpoint and need both added edges
+ * within grid_length.  props[i] = (a, b, c, d, r) with (a, b) = line t0 and
+ * (c, d) = line t1; lines_out[i] = (t0, t1, valid).  Exposed for tests. */
 int gg_sa_proposals(int nodes, int degree, const int32_t *edges, const int32_t *adj, int count,
-                    uint64_t seed, uint64_t step, int32_t *props, int32_t *lines_out);
+                    uint64_t seed, uint64_t step, int grid_height, int grid_length,
+                    int32_t *props, int32_t *lines_out);
 
 /* Applies one logged move (line0, line1, variant) to an edge list the way
  * gg_sa_batched does (for replays). */

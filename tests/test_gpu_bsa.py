@@ -215,3 +215,30 @@ def test_gpu_device_proposals_are_valid_two_opt_moves(n, d):
     assert np.array_equal(props, props2) and np.array_equal(lines, lines2)
     props3, _ = SA.sa_proposals(e, n, d, count, seed=3, step=8)
     assert not np.array_equal(props, props3)
+
+
+def test_gpu_device_grid_proposals_are_local_valid_moves():
+    """Grid draws: line t1 has an endpoint within the length of line t0's
+    first endpoint, and `valid` is exactly "both added edges are new and
+    within the length"."""
+    w, h, d, r = 30, 30, 4, 2
+    n = w * h
+    e = G.random_grid(w, h, d, r, 3)
+    props, lines = SA.sa_proposals(e, n, d, 20000, seed=5, step=2, grid_height=h, grid_length=r)
+    t0, t1, valid = lines.T
+    a, b, c, dd, v = props.T
+    assert np.array_equal(e[t0, 0], a) and np.array_equal(e[t0, 1], b)
+    assert np.array_equal(e[t1, 0], c) and np.array_equal(e[t1, 1], dd)
+
+    def dist(x, y):
+        return np.abs(x // h - y // h) + np.abs(x % h - y % h)
+    assert np.all((dist(a, c) <= r) | (dist(a, dd) <= r))
+    keys = set(map(tuple, np.sort(e, axis=1).tolist()))
+    q1 = np.where(v == 1, c, dd)
+    p2 = np.where(v == 1, b, c)
+    q2 = np.where(v == 1, dd, b)
+    fresh = np.array([(min(x, y), max(x, y)) not in keys and (min(u, z), max(u, z)) not in keys
+                      for x, y, u, z in zip(a, q1, p2, q2)])
+    want = fresh & (dist(a, q1) <= r) & (dist(p2, q2) <= r)
+    assert np.array_equal(valid == 1, want)
+    assert valid.mean() > 0.2
