@@ -133,7 +133,8 @@ def sa_time_to_aspl(budget_s):
                         use_surrogate=False, exact=1, seed=1, incremental=True)
     t_inc = float(inc.stats["seconds_to_target"])
     sur = SA.sa_batched(e0, n, d, max_steps=10**9, time_budget_s=budget_s, target_aspl=target,
-                        proposals=12288, combine=128, warmup=100, explore_every=16, seed=1)
+                        proposals=12288, combine=128, warmup=100, explore_every=16, seed=1,
+                        incremental=True)
     t_sur = float(sur.stats["seconds_to_target"])
     return {"workload": "sa n=65536 d=16 (random regular start, seed 5)",
             "target_aspl": target, "plain_seconds": round(t_plain, 3),

@@ -661,9 +661,10 @@ extern "C" int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa
     ~IncrGuard() { gg_incr_destroy(p); }
   } incr;
   bool incr_pending = false;
+  // With the surrogate, the scorer serves the explore steps until the first
+  // ranked step (merged candidates would flag nearly every source).
   if (P->incremental) {
-    if (use_sur) return set_error(GG_ERR_INVALID, "incremental scoring is for runs without the surrogate");
-    incr.p = gg_incr_create(nodes, degree, g.adj.data());
+    incr.p = gg::incr_create_unchecked(nodes, degree, g.adj.data());
     if (!incr.p) return GG_ERR_INVALID;
   }
   // The move of a single-move candidate as (removed, added) edges.
@@ -895,6 +896,10 @@ extern "C" int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa
           patches[((size_t)c * max_patch + 4 * k + j) * 2 + 1] = cand[c].sl[k].val[j];
         }
     const double te = now_s();
+    if (incr.p && !explore) {  // the first ranked step retires the scorer
+      gg_incr_destroy(incr.p);
+      incr.p = nullptr;
+    }
     if (incr.p) {
       // One candidate stays pending for the accept decision; with several,
       // each is scored and dropped, and the accepted one is scored again.
@@ -1056,9 +1061,8 @@ extern "C" int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa
           if ((rc = gg_incr_accept(incr.p))) return rc;
           incr_pending = false;
           t_exact += now_s() - ti;
-        } else if ((rc = gg_plan_set_base(plan.p, g.adj.data()))) {
-          return rc;
         }
+        if ((use_sur || !incr.p) && (rc = gg_plan_set_base(plan.p, g.adj.data()))) return rc;
         cur_diam = nd;
         cur_S = nS;
         ++accepts;

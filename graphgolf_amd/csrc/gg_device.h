@@ -4,7 +4,10 @@
 #define GG_DEVICE_H
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
+
 struct gg_mlp;
+struct gg_incr;
 
 namespace gg {
 // The evaluator's stream (adjacency uploads, plans, histograms run on it).
@@ -13,6 +16,9 @@ hipStream_t engine_stream();
 // graph: threads per block, vertices per thread, reach-array capacity, queue
 // capacity and dynamic LDS bytes (the evaluator's own choice for that n).
 int tile_shape(int bits, int n, int *block, int *vpt, int *n_cap, int *qcap, size_t *lds);
+// gg_incr_create without the graph checks, for callers that hold a simple
+// regular graph already (the batched SA validates its graph on entry).
+gg_incr *incr_create_unchecked(int nodes, int degree, const int32_t *adj);
 // Forward pass on m rows already in device memory (row-major, in_dim floats
 // each); copies the m predictions to y_host and waits for them.
 int mlp_forward_device(gg_mlp *M, const float *x_dev, int m, float *y_host);

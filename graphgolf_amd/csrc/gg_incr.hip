@@ -382,6 +382,16 @@ gg_incr *gg_incr_create(int nodes, int degree, const int32_t *adj) {
     return nullptr;
   }
   if (check_graph(nodes, degree, adj)) return nullptr;
+  return gg::incr_create_unchecked(nodes, degree, adj);
+}
+
+}  // extern "C"
+
+gg_incr *gg::incr_create_unchecked(int nodes, int degree, const int32_t *adj) {
+  if (!adj || nodes < 3 || nodes > 65536 || degree < 2 || degree > kWord) {
+    set_error(GG_ERR_INVALID, "incremental scoring needs 3 <= nodes <= 65536, 2 <= degree <= 16");
+    return nullptr;
+  }
   if (gg_init(-1)) return nullptr;
   gg_incr *s = new gg_incr();
   auto fail = [&]() -> gg_incr * {
@@ -509,6 +519,8 @@ gg_incr *gg_incr_create(int nodes, int degree, const int32_t *adj) {
   s->diameter = diam;
   return s;
 }
+
+extern "C" {
 
 int gg_incr_state(const gg_incr *s, gg_result *out) {
   if (!s || !out) return set_error(GG_ERR_INVALID, "null argument");

@@ -255,9 +255,11 @@ typedef struct gg_bsa_params {
   int32_t log_capacity;   /* entries in the log arrays */
   int32_t grid_height;    /* > 0: a grid graph, vertex v at (v / height, v % height) */
   int32_t grid_length;    /* grid: max Manhattan length of every edge (grid/sa.c:196-201) */
-  int32_t incremental;    /* without the surrogate: score candidates with the incremental
-                             scorer (include/gg_incr.h; simple graphs, degree <= 16) --
-                             same exact results, so the same run, faster at large n */
+  int32_t incremental;    /* score single-move candidates with the incremental scorer
+                             (include/gg_incr.h; simple graphs, degree <= 16) -- same
+                             exact results, faster at large n.  Without the surrogate:
+                             every step; with it: the explore steps up to the first
+                             ranked step (merged candidates flag nearly every source) */
   int32_t pad;
 } gg_bsa_params;
 

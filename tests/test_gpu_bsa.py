@@ -242,3 +242,19 @@ def test_gpu_device_grid_proposals_are_local_valid_moves():
     want = fresh & (dist(a, q1) <= r) & (dist(p2, q2) <= r)
     assert np.array_equal(valid == 1, want)
     assert valid.mean() > 0.2
+
+
+def test_gpu_surrogate_sa_incremental_warmup_is_the_same_run():
+    """The surrogate run with its explore steps scored incrementally (until
+    the first ranked step) makes the same run as with full evaluations."""
+    n, d = 4096, 16
+    e0 = G.random_regular(n, d, 12)
+    kw = dict(max_steps=80, proposals=2048, combine=32, warmup=40, explore_every=8, seed=3,
+              train_steps=1, batch_train=128)
+    full = SA.sa_batched(e0, n, d, **kw)
+    inc = SA.sa_batched(e0, n, d, incremental=True, **kw)
+    assert full.stats["accepts"] > 0
+    assert np.array_equal(full.accepted, inc.accepted)
+    assert np.array_equal(full.accepted_diam, inc.accepted_diam)
+    assert np.array_equal(full.accepted_sum, inc.accepted_sum)
+    assert np.array_equal(full.edges, inc.edges)
