@@ -30,6 +30,7 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBPS = 8000.0
+LDS_PEAK_GBPS = 256 * 128 * 2.4  # 256 CUs x 128 B/clk x 2.4 GHz
 METRIC = "APSP evals/sec + achieved GB/s; SA time-to-ASPL, n=65536 d=16 on 1 GPU"
 
 WORKLOADS = {
@@ -208,16 +209,22 @@ def incremental_bench(args):
     print(json.dumps(line), flush=True)
 
 
-def read_traffic(workload):
-    """HBM bytes per tile-BFS launch from the committed rocprofv3 PMC summary."""
-    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.md")
+def read_profile_value(name, key):
+    """A figure (`key: value` line) from a committed rocprofv3 summary."""
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     with open(path) as f:
         for line in f:
-            if line.startswith("hbm_bytes_per_launch:"):
-                return int(float(line.split(":", 1)[1]))
+            if line.startswith(key + ":"):
+                return float(line.split(":", 1)[1])
     return None
+
+
+def read_traffic(workload):
+    """HBM bytes per tile-BFS launch from the committed rocprofv3 PMC summary."""
+    v = read_profile_value(f"traffic_{workload}.md", "hbm_bytes_per_launch")
+    return None if v is None else int(v)
 
 
 def main():
@@ -312,6 +319,14 @@ def main():
         "cpu_baseline": cpu,
         "host_adjacency_ms_per_eval": handover_ms,
     }
+    # LDS, the binding memory: bytes per launch from the committed rocprofv3
+    # summary over this run's kernel time, against 256 CUs x 128 B/clk x 2.4 GHz.
+    lds_bytes = read_profile_value(f"lds_{args.workload}.md", "lds_bytes_per_launch")
+    if lds_bytes is not None:
+        lds_gbps = lds_bytes / (k_ms * 1e-3) / 1e9
+        line["lds"] = {"achieved": round(lds_gbps, 1), "peak": LDS_PEAK_GBPS, "unit": "GB/s",
+                       "frac": round(lds_gbps / LDS_PEAK_GBPS, 4), "bytes": int(lds_bytes),
+                       "busy_frac": read_profile_value(f"lds_{args.workload}.md", "lds_busy_frac")}
     plan.close()
     if args.workload == "c5" and args.sa_budget > 0:
         line["sa_time_to_aspl"] = sa_time_to_aspl(args.sa_budget)
