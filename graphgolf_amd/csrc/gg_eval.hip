@@ -171,6 +171,230 @@ gg_patch_kernel(const int32_t *__restrict__ patches, int max_patch, int count,
 }
 
 
+// ---------------------------------------------------------------------------
+// Bank-aware order of the rows.  A pull level of the tile BFS reads entry
+// slot j of 32 consecutive rows with one LDS read per half-wave (vertex
+// r*BLOCK + t, BLOCK a multiple of 64), and the LDS serves a half-wave from
+// 32 banks of 4 bytes: ids whose reach words lie in one bank but in
+// different 4-byte words are served one after another.  Random ids put
+// about 3.5 words on the busiest bank of a slot; a row's entries can be
+// permuted freely (the OR over a row does not depend on its order), so this
+// kernel orders every row of a 32-row group slot by slot to spread each
+// slot over the banks:
+//   * capacity plan: with D the largest number of group entries on one bank
+//     and dp slots, kh = D - clo*dp slots allow clo+1 rows per bank and the
+//     rest clo, clo = max(1, D/dp) -- enough for every entry;
+//   * per slot, every row takes its unused entry on the bank with the most
+//     unused group entries, LDS tickets deciding between rows on one bank;
+//     a row turned away takes the bank with room and the most unused
+//     entries, and after kOrderRounds its least-loaded bank.
+// The given order is then costed exactly (sum over slots of the largest
+// number of distinct words on one bank) and kept per group unless the new
+// order's slot loads (an upper bound of its cost) are lower, so structured
+// graphs whose given order is already conflict-free keep it.
+// Each wave orders two groups (its half-waves); a block holds four waves.
+struct OrderDesc {
+  uint16_t *adjR;
+  unsigned long long *cost;  // [2]: original, kept (nullable)
+  int n, dp, shift;          // word index of id u = (u << shift) >> 2
+};
+
+// Per-slot bank counters: bits 0-7 tickets handed out, 8-15 rows that
+// overflowed onto the bank (at most 32 per round and half-wave).  Bank 32
+// of the arrays is a sink for used entries: no unused entries, no room and
+// a load no real bank reaches.
+constexpr int kOrderRounds = 3;
+constexpr int kOrderWaves = 4;  // one wave per SIMD; the waves share nothing
+constexpr uint32_t kTicket = 1u, kOverflow = 1u << 8, kSink = 0xffffu;
+
+template <int DP> struct OrderLds {  // one wave's share
+  uint32_t degr[2][33];    // unused group entries per bank
+  uint32_t tk[DP][2][33];  // tickets | overflows << 8, per slot
+  __attribute__((aligned(16))) uint32_t cnt[DP][2][32];
+  uint32_t bm[2048];       // one bit per 4-byte word (ids < 65536, words up to 4 bytes)
+  uint16_t rowL[64][DP];   // the given row
+  uint16_t nr[64][DP];     // the new row
+};
+
+template <int NCH>
+__global__ void __launch_bounds__(64 * kOrderWaves)
+gg_order_rows_kernel(const OrderDesc *__restrict__ descs) {
+  constexpr int DP = 8 * NCH;
+  const OrderDesc P = descs[blockIdx.y];
+  if (P.dp != DP || P.shift < 0) return;  // uniform
+  __shared__ OrderLds<DP> lds_all[kOrderWaves];
+  const int lane = threadIdx.x & 63, h = lane >> 5, L = lane & 31;
+  OrderLds<DP> &S = lds_all[threadIdx.x >> 6];
+  const int v = ((int)blockIdx.x * kOrderWaves + (int)(threadIdx.x >> 6)) * 64 + lane;
+  if (__builtin_amdgcn_readfirstlane(v - lane) >= P.n) return;  // whole wave past n
+  const bool live = v < P.n;
+  const int sh = P.shift;
+  // Groups never cross waves, so a wave barrier (LDS operations of one wave
+  // complete in order) is all the synchronisation needed.
+  auto wsync = [] {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  };
+  uint32_t wd[DP];  // 4-byte LDS word of each entry's reach word; bank = wd & 31
+  {
+    const Chunk *row = (const Chunk *)(P.adjR + (size_t)(live ? v : 0) * DP);
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+      const Chunk x = live ? row[c] : Chunk{0u, 0u, 0u, 0u};
+      const uint32_t q[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        wd[8 * c + 2 * k] = ((q[k] & 0xffffu) << sh) >> 2;
+        wd[8 * c + 2 * k + 1] = ((q[k] >> 16) << sh) >> 2;
+        S.rowL[lane][8 * c + 2 * k] = (uint16_t)q[k];
+        S.rowL[lane][8 * c + 2 * k + 1] = (uint16_t)(q[k] >> 16);
+      }
+    }
+  }
+  for (int k = lane; k < DP * 2 * 33; k += 64) (&S.tk[0][0][0])[k] = k % 33 == 32 ? kSink : 0u;
+  for (int k = lane; k < DP * 64; k += 64) (&S.cnt[0][0][0])[k] = 0u;
+  for (int k = lane; k < 2048; k += 64) S.bm[k] = 0u;
+  if (lane < 2) S.degr[lane][32] = 0u;
+  S.degr[h][L] = 0u;
+  wsync();
+  uint32_t bk[DP];  // bank of entry i; 32 once used
+#pragma unroll
+  for (int i = 0; i < DP; ++i) {
+    bk[i] = live ? wd[i] & 31u : 32u;
+    if (live) atomicAdd(&S.degr[h][bk[i]], 1u);
+  }
+  wsync();
+  int dmax = (int)S.degr[h][L];
+#pragma unroll
+  for (int off = 16; off >= 1; off >>= 1) dmax = max(dmax, __shfl_xor(dmax, off, 32));
+  const int clo = max(1, dmax / DP), kh = max(0, dmax - clo * DP);
+
+  for (int s = 0; s < DP; ++s) {
+    const uint32_t cap = (uint32_t)(clo + (s < kh ? 1 : 0));
+    // Round 1: every bank has room; the entry on the bank with the most
+    // unused entries (key = degree << 5 | entry).
+    uint32_t kd[DP];
+    uint32_t best = 0;
+#pragma unroll
+    for (int i = 0; i < DP; ++i) {
+      kd[i] = (S.degr[h][bk[i]] << 5) | (uint32_t)i;
+      best = max(best, kd[i]);
+    }
+    uint32_t bi = best & 31u;
+    uint32_t pick = S.rowL[lane][bi];
+    uint32_t bw = (pick << sh) >> 2;
+    bool placed = !live;
+    if (live) placed = (atomicAdd(&S.tk[s][h][bw & 31u], kTicket) & 255u) < cap;
+    wsync();
+    // Later rounds for the rest: the bank with room and the most unused
+    // entries, else (and in the last round always) the least-loaded bank.
+    for (int round = 1; round <= kOrderRounds; ++round) {
+      if (__builtin_amdgcn_ballot_w64(!placed) == 0ull) break;
+      if (!placed) {
+        // All counters first, then branch-free keys: a branch per entry
+        // would wait for each read in turn.
+        const uint32_t not_last = round == kOrderRounds ? 0u : ~0u;
+        uint32_t tt[DP];
+#pragma unroll
+        for (int i = 0; i < DP; ++i) tt[i] = S.tk[s][h][bk[i]];
+        best = 0;
+#pragma unroll
+        for (int i = 0; i < DP; ++i) {
+          const uint32_t tick = tt[i] & 255u, load = min(min(tick, cap) + (tt[i] >> 8), kSink);
+          const uint32_t k_full = (1u << 30) | ((kSink - load) << 5) | (uint32_t)i;
+          const uint32_t k_room = (2u << 30) | kd[i];
+          const uint32_t room = not_last & (0u - (uint32_t)(tick < cap));
+          best = max(best, k_full ^ ((k_full ^ k_room) & room));
+        }
+        bi = best & 31u;
+        pick = S.rowL[lane][bi];
+        bw = (pick << sh) >> 2;
+        if (best >= (2u << 30)) {
+          placed = (atomicAdd(&S.tk[s][h][bw & 31u], kTicket) & 255u) < cap;
+        } else {
+          atomicAdd(&S.tk[s][h][bw & 31u], kOverflow);
+          placed = true;
+        }
+      }
+      wsync();
+    }
+    if (live) {
+#pragma unroll
+      for (int i = 0; i < DP; ++i) bk[i] = (uint32_t)i == bi ? 32u : bk[i];
+      atomicSub(&S.degr[h][bw & 31u], 1u);
+      S.nr[lane][s] = (uint16_t)pick;
+    }
+    wsync();
+  }
+
+  // Exact cost of an order: per slot, a lane counts its word if it is the
+  // first of its half to set the word's bit (LDS atomic OR, one half-wave
+  // at a time; the bit is cleared again straight after); the slot's cost
+  // is the most counted words on one bank.  Uniform per half-wave.
+  auto order_cost = [&](const uint16_t (*rw)[DP]) -> int {
+#pragma unroll
+    for (int s = 0; s < DP; ++s) {
+      const uint32_t x = ((uint32_t)rw[lane][s] << sh) >> 2;
+#pragma unroll
+      for (int half = 0; half < 2; ++half) {
+        if (live && h == half) {
+          const uint32_t old = atomicOr(&S.bm[x >> 5], 1u << (x & 31u));
+          if (!(old & (1u << (x & 31u)))) atomicAdd(&S.cnt[s][h][x & 31u], 1u);
+          S.bm[x >> 5] = 0u;  // every lane of this half with this word has set it by now
+        }
+        wsync();
+      }
+    }
+    int m = 0;
+    if (L < DP) {
+#pragma unroll
+      for (int j = 0; j < 32; j += 4) {
+        const Chunk c = *(const Chunk *)&S.cnt[L][h][j];
+        m = max(m, (int)max(max(c.x, c.y), max(c.z, c.w)));
+      }
+    }
+#pragma unroll
+    for (int off = 16; off >= 1; off >>= 1) m += __shfl_xor(m, off, 32);
+    wsync();
+    for (int k = lane; k < DP * 64; k += 64) (&S.cnt[0][0][0])[k] = 0u;
+    wsync();
+    return m;
+  };
+  // The new order's slot loads (rows per bank, so an upper bound of its
+  // cost) decide; its exact cost is only needed when it is reported.
+  int bound = 0;
+  if (L < DP) {
+    const uint32_t cap = (uint32_t)(clo + (L < kh ? 1 : 0));
+#pragma unroll
+    for (int j = 0; j < 32; ++j) {
+      const uint32_t t = S.tk[L][h][j];
+      bound = max(bound, (int)(min(t & 255u, cap) + (t >> 8)));
+    }
+  }
+#pragma unroll
+  for (int off = 16; off >= 1; off >>= 1) bound += __shfl_xor(bound, off, 32);
+  const int cost_old = order_cost(S.rowL);
+  const bool keep_new = bound < cost_old;  // uniform within the half-wave
+  const int cost_new = P.cost ? order_cost(S.nr) : bound;
+  if (live && keep_new) {
+    Chunk *row = (Chunk *)(P.adjR + (size_t)v * DP);
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+      Chunk x;
+      x.x = (uint32_t)S.nr[lane][8 * c + 0] | ((uint32_t)S.nr[lane][8 * c + 1] << 16);
+      x.y = (uint32_t)S.nr[lane][8 * c + 2] | ((uint32_t)S.nr[lane][8 * c + 3] << 16);
+      x.z = (uint32_t)S.nr[lane][8 * c + 4] | ((uint32_t)S.nr[lane][8 * c + 5] << 16);
+      x.w = (uint32_t)S.nr[lane][8 * c + 6] | ((uint32_t)S.nr[lane][8 * c + 7] << 16);
+      row[c] = x;
+    }
+  }
+  if (P.cost && L == 0) {
+    atomicAdd(&P.cost[0], (unsigned long long)cost_old);
+    atomicAdd(&P.cost[1], (unsigned long long)(keep_new ? cost_new : cost_old));
+  }
+}
+
 // Per-graph combination of the unit partial sums
 // (general/evaluation.c:199-200, :241, :247-249 as integers).  One
 // workgroup per graph; its units are contiguous.
@@ -507,6 +731,7 @@ struct Assembly {
   std::vector<GraphMeta> meta;
   std::vector<LaunchGroup> groups;
   size_t adj_elems = 0, adjT_elems = 0, rel_elems = 0;
+  int order_nch_mask = 0, order_max_n = 0;  // set by upload_order
 };
 
 int validate(const gg_graph &g) {
@@ -598,11 +823,11 @@ int assemble(int count, const gg_graph *gs, int hist_maxd, Assembly &A) {
 
 // Device buffers of one launch set (either the engine's single scratch or a plan's own).
 struct DevSet {
-  DevBuf adj_in, relabel_in, adjT, flags, gdesc, units, outs, meta, gout, hist, prep;
+  DevBuf adj_in, relabel_in, adjT, flags, gdesc, units, outs, meta, gout, hist, prep, order;
   PinBuf res_h;
   void release() {
     for (DevBuf *b : {&adj_in, &relabel_in, &adjT, &flags, &gdesc, &units, &outs, &meta, &gout, &hist,
-                      &prep})
+                      &prep, &order})
       b->release();
     res_h.release();
   }
@@ -648,6 +873,8 @@ int prepare_buffers(Assembly &A, DevSet &D, int hist_maxd, int32_t *hist_dev,
   return GG_OK;
 }
 
+int upload_order(Assembly &A, DevSet &D);
+
 // Per-graph prep descriptors for gg_prep_kernel (uploaded before enqueue, so
 // that a captured sequence holds no host-to-device copy of them).
 int upload_prep(Assembly &A, DevSet &D, const int32_t *const *adj_src, const int32_t *const *rel_src,
@@ -664,7 +891,52 @@ int upload_prep(Assembly &A, DevSet &D, const int32_t *const *adj_src, const int
   GG_HIP(hipMemcpyAsync(D.prep.p, h.data(), (size_t)count * sizeof(PrepDesc), hipMemcpyHostToDevice,
                         E.stream));
   GG_HIP(hipStreamSynchronize(E.stream));  // h is a local
+  return upload_order(A, D);
+}
+
+// Row-order descriptors (gg_order_rows_kernel), uploaded with the prep
+// descriptors.  A graph is ordered when its reach words are 8, 16 or 32 bits
+// (a 64-bit read is served differently), its rows are at most four chunks
+// and it is large enough for the ordering to pay for its launch.
+int g_row_order = 1;
+constexpr int kOrderMinNodes = 8192;
+
+int order_shift(int bits) { return bits == 8 ? 0 : bits == 16 ? 1 : bits == 32 ? 2 : -1; }
+
+int upload_order(Assembly &A, DevSet &D) {
+  const int count = (int)A.slots.size();
+  std::vector<OrderDesc> h(count);
+  A.order_nch_mask = 0;
+  A.order_max_n = 0;
+  for (int i = 0; i < count; ++i) {
+    const Slot &s = A.slots[i];
+    int shift = order_shift(s.bits);
+    if (!g_row_order || s.n < kOrderMinNodes || s.dp > 32) shift = -1;
+    h[i] = {(uint16_t *)D.adjT.p + s.adjT_off, nullptr, s.n, s.dp, shift};
+    if (shift >= 0) {
+      A.order_nch_mask |= 1 << (s.dp / 8);
+      A.order_max_n = std::max(A.order_max_n, s.n);
+    }
+  }
+  if (!A.order_nch_mask) return GG_OK;
+  int rc = D.order.ensure((size_t)count * sizeof(OrderDesc));
+  if (rc) return rc;
+  GG_HIP(hipMemcpyAsync(D.order.p, h.data(), (size_t)count * sizeof(OrderDesc), hipMemcpyHostToDevice,
+                        E.stream));
+  GG_HIP(hipStreamSynchronize(E.stream));  // h is a local
   return GG_OK;
+}
+
+void launch_order(const OrderDesc *descs, int count, int max_n, int nch_mask, hipStream_t st) {
+  const unsigned gx = (unsigned)((max_n + 64 * kOrderWaves - 1) / (64 * kOrderWaves));
+  for (int g0 = 0; g0 < count; g0 += 65535) {  // gridDim.y is at most 65535
+    const dim3 grid(gx, std::min(65535, count - g0));
+    const dim3 block(64 * kOrderWaves);
+    if (nch_mask & 2) hipLaunchKernelGGL(gg_order_rows_kernel<1>, grid, block, 0, st, descs + g0);
+    if (nch_mask & 4) hipLaunchKernelGGL(gg_order_rows_kernel<2>, grid, block, 0, st, descs + g0);
+    if (nch_mask & 8) hipLaunchKernelGGL(gg_order_rows_kernel<3>, grid, block, 0, st, descs + g0);
+    if (nch_mask & 16) hipLaunchKernelGGL(gg_order_rows_kernel<4>, grid, block, 0, st, descs + g0);
+  }
 }
 
 // Enqueue prep + tile BFS + combine for an assembled set whose adjacency
@@ -710,6 +982,8 @@ int enqueue(Assembly &A, DevSet &D, const int32_t *const *adj_src, const int32_t
       hipLaunchKernelGGL(gg_prep_kernel, dim3((max_entries + 255) / 256, std::min(65535, count - g0)),
                          dim3(256), 0, E.stream, (const PrepDesc *)D.prep.p + g0);
   }
+  if (A.order_nch_mask)
+    launch_order((const OrderDesc *)D.order.p, count, A.order_max_n, A.order_nch_mask, E.stream);
   if (ev_begin) GG_HIP(hipEventRecord(ev_begin, E.stream));
   for (const LaunchGroup &lg : A.groups) {
     TileKernel k = kernel_for(lg.key);
@@ -898,6 +1172,60 @@ int gg_trace_levels(const gg_graph *g, int units, uint64_t *trace, gg_result *ou
   return rc;
 }
 
+int gg_set_row_order(int on) {
+  const int prev = g_row_order;
+  g_row_order = on ? 1 : 0;
+  return prev;
+}
+
+int gg_row_order(int nodes, int degree, const int32_t *adj, int word_bits, int32_t *rows,
+                 uint64_t *cost) {
+  if (nodes < 2 || nodes > 65536 || degree < 1 || !adj || !rows || !cost)
+    return set_error(GG_ERR_INVALID, "bad row-order arguments");
+  if (word_bits != 8 && word_bits != 16 && word_bits != 32 && word_bits != 64)
+    return set_error(GG_ERR_INVALID, "word bits must be 8/16/32/64");
+  int rc = gg::init(-1);
+  if (rc) return rc;
+  const int dp = (degree + 7) / 8 * 8;
+  const size_t entries = (size_t)nodes * dp;
+  DevBuf adj_d, rows_d, flags_d, desc_d, cost_d;
+  auto done = [&](int r) {
+    for (DevBuf *b : {&adj_d, &rows_d, &flags_d, &desc_d, &cost_d}) b->release();
+    return r;
+  };
+  if ((rc = adj_d.ensure((size_t)nodes * degree * 4)) || (rc = rows_d.ensure(entries * 2 + 64)) ||
+      (rc = flags_d.ensure(8)) || (rc = desc_d.ensure(sizeof(PrepDesc) + sizeof(OrderDesc))) ||
+      (rc = cost_d.ensure(16)))
+    return done(rc);
+  hipStream_t st = engine().stream;
+  const int shift = dp <= 32 ? order_shift(word_bits) : -1;
+  const PrepDesc pd = {(const int32_t *)adj_d.p, nullptr, (uint16_t *)rows_d.p, (int32_t *)flags_d.p,
+                       nodes, degree, dp, 0};
+  const OrderDesc od = {(uint16_t *)rows_d.p, (unsigned long long *)cost_d.p, nodes, dp, shift};
+  const uint64_t zero[2] = {0, 0};
+  std::vector<uint16_t> h(entries);
+  int32_t flags[2] = {0, 0};
+  if (hipMemcpyAsync(adj_d.p, adj, (size_t)nodes * degree * 4, hipMemcpyHostToDevice, st) ||
+      hipMemcpyAsync(flags_d.p, zero, 8, hipMemcpyHostToDevice, st) ||
+      hipMemcpyAsync(cost_d.p, zero, 16, hipMemcpyHostToDevice, st) ||
+      hipMemcpyAsync(desc_d.p, &pd, sizeof pd, hipMemcpyHostToDevice, st) ||
+      hipMemcpyAsync((char *)desc_d.p + sizeof(PrepDesc), &od, sizeof od, hipMemcpyHostToDevice, st))
+    return done(set_error(GG_ERR_HIP, "row-order upload failed"));
+  hipLaunchKernelGGL(gg_prep_kernel, dim3((unsigned)((entries + 255) / 256), 1), dim3(256), 0, st,
+                     (const PrepDesc *)desc_d.p);
+  if (shift >= 0)
+    launch_order((const OrderDesc *)((const char *)desc_d.p + sizeof(PrepDesc)), 1, nodes,
+                 1 << (dp / 8), st);
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpyAsync(h.data(), rows_d.p, entries * 2, hipMemcpyDeviceToHost, st) ||
+      hipMemcpyAsync(flags, flags_d.p, 8, hipMemcpyDeviceToHost, st) ||
+      hipMemcpyAsync(cost, cost_d.p, 16, hipMemcpyDeviceToHost, st) || hipStreamSynchronize(st))
+    return done(set_error(GG_ERR_HIP, "row-order launch failed"));
+  if (flags[0]) return done(set_error(GG_ERR_INVALID, "adjacency has %d out-of-range entries", flags[0]));
+  for (size_t k = 0; k < entries; ++k) rows[k] = h[k];
+  return done(GG_OK);
+}
+
 int gg_set_word_bits(int bits) {
   if (bits != 0 && bits != 8 && bits != 16 && bits != 32 && bits != 64)
     return set_error(GG_ERR_INVALID, "word bits must be 0/8/16/32/64");
@@ -1051,6 +1379,7 @@ static gg_plan *plan_create(int count, const gg_graph *shapes, int max_patch) {
   // Capture prep + BFS + combine + result copy once; replays reuse it.
   if (max_patch == 0 && upload_prep(p->A, p->D, p->src.data(), p->rel.data(), p->sym.data()))
     return plan_fail(p);
+  if (max_patch > 0 && upload_order(p->A, p->D)) return plan_fail(p);
   hipStream_t s = engine().stream;
   if (hipStreamSynchronize(s) != hipSuccess ||
       hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {

@@ -206,6 +206,27 @@ def tune_table_load(path: str) -> int:
     return n
 
 
+def set_row_order(on: bool) -> bool:
+    """Turn the bank-aware row order of plans and evaluations made afterwards
+    on or off; returns the previous setting.  Results never depend on it."""
+    return bool(L.lib().gg_set_row_order(1 if on else 0))
+
+
+def row_order(adj, word_bits: int = 16):
+    """The converted row table of one graph as an evaluation orders it for
+    `word_bits`-bit reach words: (rows [n][dp] int32, (cost of the given
+    order, cost of the returned order)), cost = sum over 32-row groups and
+    entry slots of the most distinct 4-byte LDS words on one of 32 banks."""
+    adj = L.c32(adj)
+    n, d = adj.shape
+    dp = (d + 7) // 8 * 8
+    rows = np.zeros((n, dp), dtype=np.int32)
+    cost = np.zeros(2, dtype=np.uint64)
+    L.check(L.lib().gg_row_order(n, d, adj.ctypes.data_as(L.i32p), word_bits,
+                                 rows.ctypes.data_as(L.i32p), cost.ctypes.data_as(L.u64p)))
+    return rows, (int(cost[0]), int(cost[1]))
+
+
 def set_word_bits(bits: int) -> None:
     """Force the LDS word width (8/16/32/64 sources per word); 0 = automatic."""
     L.check(L.lib().gg_set_word_bits(bits))

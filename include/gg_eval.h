@@ -140,6 +140,22 @@ int gg_distance_histogram(int nodes, int degree, const int32_t *adj, int maxd, i
  * trace[(unit*64 + k)*2 + {0,1}]. */
 int gg_trace_levels(const gg_graph *g, int units, uint64_t *trace, gg_result *out);
 
+/* ---- Bank-aware row order (no counterpart in the reference, which gathers
+ * from host memory).  Evaluations of graphs with n >= 8192 and 8/16/32-bit
+ * reach words permute the entries of every converted adjacency row so that
+ * the 32 rows a half-wave gathers together spread each entry slot over the
+ * 32 LDS banks; results do not depend on row order.  gg_set_row_order(0)
+ * turns it off for plans and evaluations made afterwards (returns the
+ * previous setting).  gg_row_order converts one graph as an evaluation
+ * does and orders it for `word_bits` (64: not ordered), returning the row
+ * table rows[nodes][dp] (dp = degree rounded up to 8, padding = the row's
+ * own id) and the conflict cost of the given and of the returned order:
+ * cost[k] = sum over 32-row groups and slots of the largest number of
+ * distinct 4-byte LDS words that one bank serves. */
+int gg_set_row_order(int on);
+int gg_row_order(int nodes, int degree, const int32_t *adj, int word_bits, int32_t *rows,
+                 uint64_t cost[2]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,110 @@
+"""Bank-aware row order of the converted adjacency (gg_order_rows_kernel).
+
+Every returned row must be a permutation of the adjacency row plus its
+padding (the row's own id), so no reach set can change; the returned costs
+must equal the LDS bank cost recomputed here from the rows (the largest
+number of distinct 4-byte words one of 32 banks serves, summed over 32-row
+groups and entry slots); the kept order never costs more than the given
+one, and an order that is already conflict-free is kept as it is.
+Evaluations with the order on and off must agree exactly.
+"""
+import numpy as np
+import pytest
+
+from graphgolf_amd import evaluation as E
+from graphgolf_amd import graphs as G
+
+pytestmark = pytest.mark.gpu
+
+SHIFT = {8: 0, 16: 1, 32: 2}
+
+
+def padded(adj):
+    n, d = adj.shape
+    dp = (d + 7) // 8 * 8
+    out = np.empty((n, dp), dtype=np.int64)
+    out[:, :d] = adj
+    out[:, d:] = np.arange(n)[:, None]
+    return out
+
+
+def host_cost(rows, word_bits):
+    words = (np.asarray(rows, dtype=np.int64) << SHIFT[word_bits]) >> 2
+    n, dp = words.shape
+    total = 0
+    for g0 in range(0, n, 32):
+        blk = words[g0:g0 + 32]
+        for s in range(dp):
+            distinct = np.unique(blk[:, s])
+            total += int(np.bincount(distinct & 31, minlength=32).max())
+    return total
+
+
+@pytest.mark.parametrize("n,d,bits,seed", [
+    (65536, 16, 16, 5),
+    (8192, 20, 8, 1),     # three chunks per row, byte words
+    (10000, 5, 32, 2),    # one chunk with padding, a partial last group
+    (9000, 32, 16, 3),    # four chunks
+])
+def test_rows_permuted_and_cost_exact(n, d, bits, seed):
+    E.init(0)
+    adj = G.edges_to_adj(n, d, G.random_regular(n, d, seed))
+    rows, (c_old, c_new) = E.row_order(adj, bits)
+    ref = padded(adj)
+    assert rows.shape == ref.shape
+    assert (np.sort(rows, axis=1) == np.sort(ref, axis=1)).all()
+    assert host_cost(ref, bits) == c_old
+    assert host_cost(rows, bits) == c_new
+    assert c_new <= c_old
+    if (n, d) == (65536, 16):
+        # random ids put ~3.5 words on a slot's busiest bank, the order ~2.1
+        assert c_new < 0.65 * c_old, (c_old, c_new)
+
+
+def test_conflict_free_order_is_kept():
+    E.init(0)
+    n, offs = 16384, [1, 2, 3, 5, 8, 13, 21, 34]
+    adj = np.array([[(v + o) % n for o in offs] for v in range(n)], dtype=np.int32)
+    rows, (c_old, c_new) = E.row_order(adj, 16)
+    assert c_old == (n // 32) * 8  # one word per bank in every slot
+    assert c_new == c_old
+    assert (rows == padded(adj)).all()
+
+
+def test_word64_is_not_reordered():
+    E.init(0)
+    adj = G.edges_to_adj(8192, 16, G.random_regular(8192, 16, 4))
+    rows, cost = E.row_order(adj, 64)
+    assert (rows == padded(adj)).all()
+    assert cost == (0, 0)
+
+
+@pytest.mark.parametrize("n,d,bits", [
+    (65536, 16, 0), (8192, 16, 8), (12288, 16, 32), (10000, 5, 16), (8200, 24, 16), (9000, 32, 32),
+])
+def test_evaluation_same_with_order_on_and_off(n, d, bits):
+    E.init(0)
+    adj = G.edges_to_adj(n, d, G.random_regular(n, d, 11))
+    E.set_word_bits(bits)
+    try:
+        prev = E.set_row_order(False)
+        off = E.apsp(adj, n, 1)
+        E.set_row_order(True)
+        on = E.apsp(adj, n, 1)
+        E.set_row_order(prev)
+    finally:
+        E.set_word_bits(0)
+    assert (on.diameter, on.sum) == (off.diameter, off.sum)
+    assert on.word_bits == off.word_bits
+
+
+def test_symmetric_groups_same_with_order_on_and_off():
+    E.init(0)
+    n, d, g = 16384, 12, 4
+    adj = G.edges_to_adj(n, d, G.random_symmetric(n, d, g, 7))
+    prev = E.set_row_order(False)
+    off = E.apsp(adj, n // g, g)
+    E.set_row_order(True)
+    on = E.apsp(adj, n // g, g)
+    E.set_row_order(prev)
+    assert (on.diameter, on.sum) == (off.diameter, off.sum)
