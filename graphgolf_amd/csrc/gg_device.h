@@ -16,6 +16,11 @@ hipStream_t engine_stream();
 // graph: threads per block, vertices per thread, reach-array capacity, queue
 // capacity and dynamic LDS bytes (the evaluator's own choice for that n).
 int tile_shape(int bits, int n, int *block, int *vpt, int *n_cap, int *qcap, size_t *lds);
+// Bank-aware order (the evaluator's gg_order_rows_kernel) of one converted
+// row table [n][dp] on `st`, for reach words of `bits` bits; a no-op where the
+// evaluator would not order (64-bit words, n below its threshold, rows of
+// more than four chunks, or ordering turned off).  Waits for completion.
+int order_rows_sync(uint16_t *adjR, int n, int dp, int bits, hipStream_t st);
 // gg_incr_create without the graph checks, for callers that hold a simple
 // regular graph already (the batched SA validates its graph on entry).
 gg_incr *incr_create_unchecked(int nodes, int degree, const int32_t *adj);

@@ -939,6 +939,26 @@ void launch_order(const OrderDesc *descs, int count, int max_n, int nch_mask, hi
   }
 }
 
+int order_rows_sync(uint16_t *adjR, int n, int dp, int bits, hipStream_t st) {
+  const int shift = order_shift(bits);
+  if (!g_row_order || shift < 0 || n < kOrderMinNodes || dp > 32 || dp % 8) return GG_OK;
+  DevBuf desc;
+  int rc = desc.ensure(sizeof(OrderDesc));
+  if (rc) return rc;
+  const OrderDesc od = {adjR, nullptr, n, dp, shift};
+  if (hipMemcpyAsync(desc.p, &od, sizeof od, hipMemcpyHostToDevice, st) != hipSuccess) {
+    desc.release();
+    return set_error(GG_ERR_HIP, "row-order upload failed");
+  }
+  launch_order((const OrderDesc *)desc.p, 1, n, 1 << (dp / 8), st);
+  const hipError_t e = hipGetLastError();
+  const hipError_t w = hipStreamSynchronize(st);
+  desc.release();
+  if (e != hipSuccess || w != hipSuccess)
+    return set_error(GG_ERR_HIP, "row order failed: %s", hipGetErrorString(e != hipSuccess ? e : w));
+  return GG_OK;
+}
+
 // Enqueue prep + tile BFS + combine for an assembled set whose adjacency
 // already sits in `adj_src[i]` (device pointers) -- graph-capturable (no
 // memset nodes, no host sync).

@@ -286,6 +286,9 @@ struct gg_incr {
   KernelSet K;
   hipStream_t stream = nullptr;
   std::vector<int32_t> adj;  // current graph, host copy [n][d]
+  // Device-row column of adjacency entry (v, j): the rows are put in the
+  // evaluator's bank-aware order once, and patches keep every position.
+  std::vector<uint16_t> slot;  // [n][d]
   // Device state.
   uint16_t *adjR = nullptr;          // [n][dp] + 64 bytes (current graph; patched while pending)
   int32_t *flags = nullptr;          // [2]
@@ -486,6 +489,21 @@ gg_incr *gg::incr_create_unchecked(int nodes, int degree, const int32_t *adj) {
   std::vector<uint32_t> sums(n);
   auto upload = [&]() -> int {
     GG_HIP(hipMemcpy(s->adjR, rows.data(), rows.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+    int orc = gg::order_rows_sync(s->adjR, n, dp, kWord, s->stream);
+    if (orc) return orc;
+    GG_HIP(hipMemcpy(rows.data(), s->adjR, (size_t)n * dp * sizeof(uint16_t), hipMemcpyDeviceToHost));
+    s->slot.assign((size_t)n * d, 0);
+    for (int v = 0; v < n; ++v) {
+      uint32_t taken = 0;  // dp <= 32 whenever the rows were reordered
+      for (int j = 0; j < d; ++j) {
+        const uint16_t want = (uint16_t)adj[(size_t)v * d + j];
+        int k = 0;
+        while (k < dp && ((k < 32 && ((taken >> k) & 1u)) || rows[(size_t)v * dp + k] != want)) ++k;
+        if (k == dp) return set_error(GG_ERR_STATE, "row %d lost neighbour %d", v, (int)want);
+        if (k < 32) taken |= 1u << k;
+        s->slot[(size_t)v * d + j] = (uint16_t)k;
+      }
+    }
     GG_HIP(hipMemcpy(s->flags, zero_flags, sizeof(zero_flags), hipMemcpyHostToDevice));
     GG_HIP(hipMemcpy(s->gdesc, g, sizeof(g), hipMemcpyHostToDevice));
     GG_HIP(hipMemcpy(s->aff, all.data(), (size_t)n * sizeof(uint16_t), hipMemcpyHostToDevice));
@@ -580,7 +598,7 @@ int gg_incr_eval(gg_incr *s, const int32_t rem[4], const int32_t add[4], gg_resu
     s->pend_cols[i] = col;
     s->pend_vals[i] = nb;
     s->pend_old[i] = partner[i];
-    redo.at[i] = undo.at[i] = P[i] * dp + col;
+    redo.at[i] = undo.at[i] = P[i] * dp + s->slot[(size_t)P[i] * d + col];
     redo.val[i] = nb;
     undo.val[i] = partner[i];
   }
