@@ -895,13 +895,30 @@ int upload_prep(Assembly &A, DevSet &D, const int32_t *const *adj_src, const int
 }
 
 // Row-order descriptors (gg_order_rows_kernel), uploaded with the prep
-// descriptors.  A graph is ordered when its reach words are 8, 16 or 32 bits
-// (a 64-bit read is served differently), its rows are at most four chunks
-// and it is large enough for the ordering to pay for its launch.
+// descriptors.  A graph is ordered when its rows are at most four chunks
+// and (mode 1) it has enough source tiles for the ordering to pay, or
+// (mode 2) always.  The tile BFS's gathers grow with rows x tiles and the
+// ordering with rows alone: on C5 (n = 65536, u16) the ordering costs 37 us
+// and saves 15 % of a 1.86 ms BFS over 4096 tiles, so it breaks even near
+// 540 tiles; mode 1 asks for kOrderMinTiles.  A row table that is ordered
+// once and then patched for many evaluations (the incremental scorer) is
+// ordered from kOrderMinNodes vertices.  Banks: 8/16/32-bit words are served from 32 banks
+// of 4 bytes per half-wave (bank = 4-byte word index mod 32); 64-bit words
+// from 32 slots of 8 bytes per half-wave, keyed by the word index mod 32
+// (measured: 3.5 cycles per wave-read spread over the 32 slots, 6.7 at
+// random), so their key is the id itself, as for 32-bit words.
 int g_row_order = 1;
+constexpr int kOrderMinTiles = 1024;
 constexpr int kOrderMinNodes = 8192;
 
-int order_shift(int bits) { return bits == 8 ? 0 : bits == 16 ? 1 : bits == 32 ? 2 : -1; }
+int order_shift(int bits) { return bits == 8 ? 0 : bits == 16 ? 1 : 2; }
+
+// `size` is the graph's source-tile count, or its vertex count for a table
+// ordered once (min_size says which threshold applies).
+bool order_applies(int size, int min_size, int dp) {
+  if (dp > 32 || dp % 8) return false;
+  return g_row_order == 2 || (g_row_order == 1 && size >= min_size);
+}
 
 int upload_order(Assembly &A, DevSet &D) {
   const int count = (int)A.slots.size();
@@ -910,8 +927,8 @@ int upload_order(Assembly &A, DevSet &D) {
   A.order_max_n = 0;
   for (int i = 0; i < count; ++i) {
     const Slot &s = A.slots[i];
-    int shift = order_shift(s.bits);
-    if (!g_row_order || s.n < kOrderMinNodes || s.dp > 32) shift = -1;
+    const int shift =
+        order_applies(A.meta[i].unit_count, kOrderMinTiles, s.dp) ? order_shift(s.bits) : -1;
     h[i] = {(uint16_t *)D.adjT.p + s.adjT_off, nullptr, s.n, s.dp, shift};
     if (shift >= 0) {
       A.order_nch_mask |= 1 << (s.dp / 8);
@@ -940,8 +957,8 @@ void launch_order(const OrderDesc *descs, int count, int max_n, int nch_mask, hi
 }
 
 int order_rows_sync(uint16_t *adjR, int n, int dp, int bits, hipStream_t st) {
+  if (!order_applies(n, kOrderMinNodes, dp)) return GG_OK;
   const int shift = order_shift(bits);
-  if (!g_row_order || shift < 0 || n < kOrderMinNodes || dp > 32 || dp % 8) return GG_OK;
   DevBuf desc;
   int rc = desc.ensure(sizeof(OrderDesc));
   if (rc) return rc;
@@ -1192,9 +1209,10 @@ int gg_trace_levels(const gg_graph *g, int units, uint64_t *trace, gg_result *ou
   return rc;
 }
 
-int gg_set_row_order(int on) {
+int gg_set_row_order(int mode) {
+  if (mode < 0 || mode > 2) return set_error(GG_ERR_INVALID, "row-order mode must be 0, 1 or 2");
   const int prev = g_row_order;
-  g_row_order = on ? 1 : 0;
+  g_row_order = mode;
   return prev;
 }
 
@@ -1204,6 +1222,7 @@ int gg_row_order(int nodes, int degree, const int32_t *adj, int word_bits, int32
     return set_error(GG_ERR_INVALID, "bad row-order arguments");
   if (word_bits != 8 && word_bits != 16 && word_bits != 32 && word_bits != 64)
     return set_error(GG_ERR_INVALID, "word bits must be 8/16/32/64");
+  if (degree > 32) return set_error(GG_ERR_INVALID, "rows of more than 32 entries are not ordered");
   int rc = gg::init(-1);
   if (rc) return rc;
   const int dp = (degree + 7) / 8 * 8;
@@ -1218,7 +1237,7 @@ int gg_row_order(int nodes, int degree, const int32_t *adj, int word_bits, int32
       (rc = cost_d.ensure(16)))
     return done(rc);
   hipStream_t st = engine().stream;
-  const int shift = dp <= 32 ? order_shift(word_bits) : -1;
+  const int shift = order_shift(word_bits);
   const PrepDesc pd = {(const int32_t *)adj_d.p, nullptr, (uint16_t *)rows_d.p, (int32_t *)flags_d.p,
                        nodes, degree, dp, 0};
   const OrderDesc od = {(uint16_t *)rows_d.p, (unsigned long long *)cost_d.p, nodes, dp, shift};
@@ -1233,9 +1252,8 @@ int gg_row_order(int nodes, int degree, const int32_t *adj, int word_bits, int32
     return done(set_error(GG_ERR_HIP, "row-order upload failed"));
   hipLaunchKernelGGL(gg_prep_kernel, dim3((unsigned)((entries + 255) / 256), 1), dim3(256), 0, st,
                      (const PrepDesc *)desc_d.p);
-  if (shift >= 0)
-    launch_order((const OrderDesc *)((const char *)desc_d.p + sizeof(PrepDesc)), 1, nodes,
-                 1 << (dp / 8), st);
+  launch_order((const OrderDesc *)((const char *)desc_d.p + sizeof(PrepDesc)), 1, nodes,
+               1 << (dp / 8), st);
   if (hipGetLastError() != hipSuccess ||
       hipMemcpyAsync(h.data(), rows_d.p, entries * 2, hipMemcpyDeviceToHost, st) ||
       hipMemcpyAsync(flags, flags_d.p, 8, hipMemcpyDeviceToHost, st) ||

@@ -206,10 +206,15 @@ def tune_table_load(path: str) -> int:
     return n
 
 
-def set_row_order(on: bool) -> bool:
-    """Turn the bank-aware row order of plans and evaluations made afterwards
-    on or off; returns the previous setting.  Results never depend on it."""
-    return bool(L.lib().gg_set_row_order(1 if on else 0))
+def set_row_order(mode) -> int:
+    """Bank-aware row order of plans and evaluations made afterwards:
+    0/False off, 1/True graphs of at least 1024 source tiles and
+    incremental scorers of at least 8192 vertices (the default), 2 every graph;
+    returns the previous mode.  Results never depend on it."""
+    rc = L.lib().gg_set_row_order(int(mode))
+    if rc < 0:
+        L.check(rc)
+    return rc
 
 
 def row_order(adj, word_bits: int = 16):

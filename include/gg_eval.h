@@ -141,18 +141,20 @@ int gg_distance_histogram(int nodes, int degree, const int32_t *adj, int maxd, i
 int gg_trace_levels(const gg_graph *g, int units, uint64_t *trace, gg_result *out);
 
 /* ---- Bank-aware row order (no counterpart in the reference, which gathers
- * from host memory).  Evaluations of graphs with n >= 8192 and 8/16/32-bit
- * reach words permute the entries of every converted adjacency row so that
- * the 32 rows a half-wave gathers together spread each entry slot over the
- * 32 LDS banks; results do not depend on row order.  gg_set_row_order(0)
- * turns it off for plans and evaluations made afterwards (returns the
- * previous setting).  gg_row_order converts one graph as an evaluation
- * does and orders it for `word_bits` (64: not ordered), returning the row
- * table rows[nodes][dp] (dp = degree rounded up to 8, padding = the row's
- * own id) and the conflict cost of the given and of the returned order:
- * cost[k] = sum over 32-row groups and slots of the largest number of
- * distinct 4-byte LDS words that one bank serves. */
-int gg_set_row_order(int on);
+ * from host memory).  Evaluations permute the entries of every converted
+ * adjacency row (degree <= 32) so that the 32 rows a half-wave gathers
+ * together spread each entry slot over the LDS banks; results do not depend
+ * on row order.  gg_set_row_order(mode) applies to plans and evaluations
+ * made afterwards: 0 off, 1 graphs of at least 1024 source tiles (the
+ * default; where the ordering pays for itself) and incremental scorers of
+ * at least 8192 vertices, 2 every graph; returns the previous mode.  gg_row_order converts one graph as an
+ * evaluation does and orders it for `word_bits`-wide reach words, returning
+ * the row table rows[nodes][dp] (dp = degree rounded up to 8, padding = the
+ * row's own id) and the conflict cost of the given and of the returned
+ * order: cost[k] = sum over 32-row groups and slots of the largest number
+ * of distinct LDS words that one bank serves (bank = word index mod 32,
+ * word index = id*bits/32 for 8/16/32-bit words, the id for 64-bit ones). */
+int gg_set_row_order(int mode);
 int gg_row_order(int nodes, int degree, const int32_t *adj, int word_bits, int32_t *rows,
                  uint64_t cost[2]);
 

@@ -16,7 +16,7 @@ from graphgolf_amd import graphs as G
 
 pytestmark = pytest.mark.gpu
 
-SHIFT = {8: 0, 16: 1, 32: 2}
+SHIFT = {8: 0, 16: 1, 32: 2, 64: 2}  # word index = (id << SHIFT) >> 2; bank = index mod 32
 
 
 def padded(adj):
@@ -45,6 +45,8 @@ def host_cost(rows, word_bits):
     (8192, 20, 8, 1),     # three chunks per row, byte words
     (10000, 5, 32, 2),    # one chunk with padding, a partial last group
     (9000, 32, 16, 3),    # four chunks
+    (8192, 16, 64, 4),    # 64-bit words: keyed by the id
+    (3000, 12, 64, 6),    # small graph, padded rows
 ])
 def test_rows_permuted_and_cost_exact(n, d, bits, seed):
     E.init(0)
@@ -71,12 +73,11 @@ def test_conflict_free_order_is_kept():
     assert (rows == padded(adj)).all()
 
 
-def test_word64_is_not_reordered():
+def test_rows_wider_than_four_chunks_are_refused():
     E.init(0)
-    adj = G.edges_to_adj(8192, 16, G.random_regular(8192, 16, 4))
-    rows, cost = E.row_order(adj, 64)
-    assert (rows == padded(adj)).all()
-    assert cost == (0, 0)
+    adj = G.edges_to_adj(1024, 40, G.random_regular(1024, 40, 4))
+    with pytest.raises(Exception):
+        E.row_order(adj, 16)
 
 
 @pytest.mark.parametrize("n,d,bits", [
@@ -87,9 +88,9 @@ def test_evaluation_same_with_order_on_and_off(n, d, bits):
     adj = G.edges_to_adj(n, d, G.random_regular(n, d, 11))
     E.set_word_bits(bits)
     try:
-        prev = E.set_row_order(False)
+        prev = E.set_row_order(0)
         off = E.apsp(adj, n, 1)
-        E.set_row_order(True)
+        E.set_row_order(2)
         on = E.apsp(adj, n, 1)
         E.set_row_order(prev)
     finally:
@@ -102,9 +103,38 @@ def test_symmetric_groups_same_with_order_on_and_off():
     E.init(0)
     n, d, g = 16384, 12, 4
     adj = G.edges_to_adj(n, d, G.random_symmetric(n, d, g, 7))
-    prev = E.set_row_order(False)
+    prev = E.set_row_order(0)
     off = E.apsp(adj, n // g, g)
-    E.set_row_order(True)
+    E.set_row_order(2)
     on = E.apsp(adj, n // g, g)
     E.set_row_order(prev)
     assert (on.diameter, on.sum) == (off.diameter, off.sum)
+
+
+@pytest.mark.parametrize("n,d,bits", [(1024, 8, 64), (3000, 24, 32), (5000, 16, 0), (4096, 32, 64)])
+def test_small_graphs_same_with_order_everywhere(n, d, bits):
+    """Mode 2 orders every graph, also below the size threshold of mode 1."""
+    E.init(0)
+    adj = G.edges_to_adj(n, d, G.random_regular(n, d, 13))
+    E.set_word_bits(bits)
+    try:
+        prev = E.set_row_order(0)
+        off = E.apsp(adj, n, 1)
+        E.set_row_order(2)
+        on = E.apsp(adj, n, 1)
+        E.set_row_order(prev)
+    finally:
+        E.set_word_bits(0)
+    assert (on.diameter, on.sum, on.word_bits) == (off.diameter, off.sum, off.word_bits)
+
+
+def test_batch_same_with_order_everywhere():
+    E.init(0)
+    adjs = [G.edges_to_adj(n, d, G.random_regular(n, d, 20 + n)) for n, d in
+            [(1500, 8), (2048, 12), (9000, 16), (700, 32), (4000, 24)]]
+    prev = E.set_row_order(0)
+    off = E.eval_batch([dict(adj=a) for a in adjs])
+    E.set_row_order(2)
+    on = E.eval_batch([dict(adj=a) for a in adjs])
+    E.set_row_order(prev)
+    assert [(r.diameter, r.sum) for r in on] == [(r.diameter, r.sum) for r in off]

@@ -59,6 +59,13 @@ def test_invalid_arguments_rejected_before_device(kw):
     assert L.lib().gg_last_error()
 
 
+def test_row_order_mode_is_checked():
+    assert L.lib().gg_set_row_order(3) == L.GG_ERR_INVALID
+    assert L.lib().gg_set_row_order(-1) == L.GG_ERR_INVALID
+    prev = L.lib().gg_set_row_order(2)
+    assert L.lib().gg_set_row_order(prev) == 2
+
+
 def test_relabel_must_be_a_permutation():
     rel = np.arange(16, dtype=np.int32)
     rel[3] = 4
