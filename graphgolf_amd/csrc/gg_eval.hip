@@ -409,18 +409,33 @@ gg_combine_kernel(const GraphMeta *__restrict__ meta, const UnitDesc *__restrict
   const GraphMeta m = meta[gidx];
   uint64_t lo = 0, hi = 0, chi = 0;
   int diam = 1, ecc = 0, disc = 0, err = 0;
-  for (int k = tid; k < m.unit_count; k += 256) {
-    const int u = m.unit_begin + k;
-    const UnitOut o = outs[u];
-    if (units[u].centre) chi += o.acc_hi;
-    else {
-      lo += o.acc_lo;
-      hi += o.acc_hi;
+  // Eight records per thread in flight at a time (a single workgroup reads
+  // every unit of the graph; one load at a time is latency-bound).
+  constexpr int kBatch = 8;
+  for (int k0 = 0; k0 < m.unit_count; k0 += 256 * kBatch) {
+    UnitOut o[kBatch];
+    int cen[kBatch];
+#pragma unroll
+    for (int j = 0; j < kBatch; ++j) {
+      const int k = k0 + j * 256 + tid;
+      const bool ok = k < m.unit_count;
+      const int u = m.unit_begin + (ok ? k : 0);
+      o[j] = outs[u];
+      cen[j] = ok ? (int)units[u].centre : -1;
     }
-    diam = max(diam, o.done_level);
-    ecc = max(ecc, o.last_change);
-    disc |= o.disconnected;
-    err |= o.error;
+#pragma unroll
+    for (int j = 0; j < kBatch; ++j) {
+      if (cen[j] < 0) continue;
+      if (cen[j]) chi += o[j].acc_hi;
+      else {
+        lo += o[j].acc_lo;
+        hi += o[j].acc_hi;
+      }
+      diam = max(diam, o[j].done_level);
+      ecc = max(ecc, o[j].last_change);
+      disc |= o[j].disconnected;
+      err |= o[j].error;
+    }
   }
   s64[0][tid] = lo;
   s64[1][tid] = hi;
