@@ -209,11 +209,15 @@ constexpr uint32_t kTicket = 1u, kOverflow = 1u << 8, kSink = 0xffffu;
 
 template <int DP> struct OrderLds {  // one wave's share
   uint32_t degr[2][33];    // unused group entries per bank
-  uint32_t tk[DP][2][33];  // tickets | overflows << 8, per slot
-  __attribute__((aligned(16))) uint32_t cnt[DP][2][32];
-  uint32_t bm[2048];       // one bit per 4-byte word (ids < 65536, words up to 4 bytes)
-  uint16_t rowL[64][DP];   // the given row
-  uint16_t nr[64][DP];     // the new row
+  union {
+    uint32_t tk[DP][2][33];  // greedy: tickets | overflows << 8, per slot
+    struct {                 // costs (after the greedy's slot loads are read)
+      __attribute__((aligned(16))) uint32_t cnt[DP][2][32];
+      uint32_t bm[2048];  // one bit per 4-byte word (ids < 65536, words up to 8 bytes)
+    } c;
+  } u;
+  uint16_t rowL[64][DP];  // the given row
+  uint16_t nr[64][DP];    // the new row
 };
 
 template <int NCH>
@@ -252,9 +256,7 @@ gg_order_rows_kernel(const OrderDesc *__restrict__ descs) {
       }
     }
   }
-  for (int k = lane; k < DP * 2 * 33; k += 64) (&S.tk[0][0][0])[k] = k % 33 == 32 ? kSink : 0u;
-  for (int k = lane; k < DP * 64; k += 64) (&S.cnt[0][0][0])[k] = 0u;
-  for (int k = lane; k < 2048; k += 64) S.bm[k] = 0u;
+  for (int k = lane; k < DP * 2 * 33; k += 64) (&S.u.tk[0][0][0])[k] = k % 33 == 32 ? kSink : 0u;
   if (lane < 2) S.degr[lane][32] = 0u;
   S.degr[h][L] = 0u;
   wsync();
@@ -285,7 +287,7 @@ gg_order_rows_kernel(const OrderDesc *__restrict__ descs) {
     uint32_t pick = S.rowL[lane][bi];
     uint32_t bw = (pick << sh) >> 2;
     bool placed = !live;
-    if (live) placed = (atomicAdd(&S.tk[s][h][bw & 31u], kTicket) & 255u) < cap;
+    if (live) placed = (atomicAdd(&S.u.tk[s][h][bw & 31u], kTicket) & 255u) < cap;
     wsync();
     // Later rounds for the rest: the bank with room and the most unused
     // entries, else (and in the last round always) the least-loaded bank.
@@ -297,7 +299,7 @@ gg_order_rows_kernel(const OrderDesc *__restrict__ descs) {
         const uint32_t not_last = round == kOrderRounds ? 0u : ~0u;
         uint32_t tt[DP];
 #pragma unroll
-        for (int i = 0; i < DP; ++i) tt[i] = S.tk[s][h][bk[i]];
+        for (int i = 0; i < DP; ++i) tt[i] = S.u.tk[s][h][bk[i]];
         best = 0;
 #pragma unroll
         for (int i = 0; i < DP; ++i) {
@@ -311,9 +313,9 @@ gg_order_rows_kernel(const OrderDesc *__restrict__ descs) {
         pick = S.rowL[lane][bi];
         bw = (pick << sh) >> 2;
         if (best >= (2u << 30)) {
-          placed = (atomicAdd(&S.tk[s][h][bw & 31u], kTicket) & 255u) < cap;
+          placed = (atomicAdd(&S.u.tk[s][h][bw & 31u], kTicket) & 255u) < cap;
         } else {
-          atomicAdd(&S.tk[s][h][bw & 31u], kOverflow);
+          atomicAdd(&S.u.tk[s][h][bw & 31u], kOverflow);
           placed = true;
         }
       }
@@ -339,9 +341,9 @@ gg_order_rows_kernel(const OrderDesc *__restrict__ descs) {
 #pragma unroll
       for (int half = 0; half < 2; ++half) {
         if (live && h == half) {
-          const uint32_t old = atomicOr(&S.bm[x >> 5], 1u << (x & 31u));
-          if (!(old & (1u << (x & 31u)))) atomicAdd(&S.cnt[s][h][x & 31u], 1u);
-          S.bm[x >> 5] = 0u;  // every lane of this half with this word has set it by now
+          const uint32_t old = atomicOr(&S.u.c.bm[x >> 5], 1u << (x & 31u));
+          if (!(old & (1u << (x & 31u)))) atomicAdd(&S.u.c.cnt[s][h][x & 31u], 1u);
+          S.u.c.bm[x >> 5] = 0u;  // every lane of this half with this word has set it by now
         }
         wsync();
       }
@@ -350,14 +352,14 @@ gg_order_rows_kernel(const OrderDesc *__restrict__ descs) {
     if (L < DP) {
 #pragma unroll
       for (int j = 0; j < 32; j += 4) {
-        const Chunk c = *(const Chunk *)&S.cnt[L][h][j];
+        const Chunk c = *(const Chunk *)&S.u.c.cnt[L][h][j];
         m = max(m, (int)max(max(c.x, c.y), max(c.z, c.w)));
       }
     }
 #pragma unroll
     for (int off = 16; off >= 1; off >>= 1) m += __shfl_xor(m, off, 32);
     wsync();
-    for (int k = lane; k < DP * 64; k += 64) (&S.cnt[0][0][0])[k] = 0u;
+    for (int k = lane; k < DP * 64; k += 64) (&S.u.c.cnt[0][0][0])[k] = 0u;
     wsync();
     return m;
   };
@@ -368,12 +370,16 @@ gg_order_rows_kernel(const OrderDesc *__restrict__ descs) {
     const uint32_t cap = (uint32_t)(clo + (L < kh ? 1 : 0));
 #pragma unroll
     for (int j = 0; j < 32; ++j) {
-      const uint32_t t = S.tk[L][h][j];
+      const uint32_t t = S.u.tk[L][h][j];
       bound = max(bound, (int)(min(t & 255u, cap) + (t >> 8)));
     }
   }
 #pragma unroll
   for (int off = 16; off >= 1; off >>= 1) bound += __shfl_xor(bound, off, 32);
+  wsync();  // every lane has read the tickets: their space now holds the costs' counters
+  for (int k = lane; k < DP * 64; k += 64) (&S.u.c.cnt[0][0][0])[k] = 0u;
+  for (int k = lane; k < 2048; k += 64) S.u.c.bm[k] = 0u;
+  wsync();
   const int cost_old = order_cost(S.rowL);
   const bool keep_new = bound < cost_old;  // uniform within the half-wave
   const int cost_new = P.cost ? order_cost(S.nr) : bound;
