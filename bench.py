@@ -240,8 +240,8 @@ def main():
                     help="seconds of plain GPU SA for the time-to-ASPL comparison (c5 only; 0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build evaluates on a single MI355X "
-              "(multi-GPU sharding is described in DESIGN.md, not run).", file=sys.stderr)
+        print("bench.py measures one GPU: the evaluator runs on a single MI355X "
+              "(sharding source tiles across GPUs is not run here).", file=sys.stderr)
         raise SystemExit(2)
 
     from graphgolf_amd import evaluation as E

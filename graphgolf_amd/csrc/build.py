@@ -24,8 +24,9 @@ INC = os.path.join(ROOT, "include")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("GG_OFFLOAD_ARCH", "gfx950")
 
-# Packed-FP32 forms are disabled for all device code (see the build notes in
-# DESIGN.md); hipcc's host pass warns that it does not know the feature.
+# Packed-FP32 instruction forms (v_pk_*_f32) are disabled for all device code:
+# the compiler's float paths (the MLP) then use scalar FP32 instructions.
+# hipcc's host pass warns that it does not know the feature.
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-I", INC, "-I", HERE,
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall",
              "-Wno-unused-function"]
