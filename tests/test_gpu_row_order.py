@@ -138,3 +138,36 @@ def test_batch_same_with_order_everywhere():
     on = E.eval_batch([dict(adj=a) for a in adjs])
     E.set_row_order(prev)
     assert [(r.diameter, r.sum) for r in on] == [(r.diameter, r.sum) for r in off]
+
+
+def test_captured_plan_replays_with_order():
+    """The order kernel runs inside the plan's captured graph: every replay,
+    with eager evaluations in between, equals the eager result."""
+    E.init(0)
+    n, d = 16384, 16  # 1024 tiles of 16 sources: ordered by default
+    adjs = [G.edges_to_adj(n, d, G.random_regular(n, d, s)) for s in (21, 22)]
+    want = [E.apsp(a, symmetric=True) for a in adjs]
+    plan = E.Plan([dict(nodes=n, degree=d, based_nodes=n, groups=1, relabel=None, symmetric=True)])
+    try:
+        for rep in range(4):
+            k = rep % 2
+            plan.set_adj(0, adjs[k])
+            got = plan.run()[0]
+            assert (got.diameter, got.sum) == (want[k].diameter, want[k].sum), rep
+            eager = E.apsp(adjs[1 - k], symmetric=True)  # eager work between replays
+            assert (eager.diameter, eager.sum) == (want[1 - k].diameter, want[1 - k].sum)
+    finally:
+        plan.close()
+
+
+def test_relabelled_graph_same_with_order_everywhere():
+    E.init(0)
+    n, d = 9000, 12
+    adj = G.edges_to_adj(n, d, G.random_regular(n, d, 23))
+    perm = np.random.default_rng(3).permutation(n).astype(np.int32)
+    prev = E.set_row_order(0)
+    off = E.apsp(adj, relabel=perm, symmetric=True)
+    E.set_row_order(2)
+    on = E.apsp(adj, relabel=perm, symmetric=True)
+    E.set_row_order(prev)
+    assert (on.diameter, on.sum) == (off.diameter, off.sum)
