@@ -1220,6 +1220,35 @@ int gg_distance_histogram(int nodes, int degree, const int32_t *adj, int maxd, i
   return eval_host(1, &g, &r, false, maxd, hist);
 }
 
+int gg_eval_features(int nodes, int degree, const int32_t *adj, int maxd, int32_t *ecc,
+                     int32_t *hist, gg_result *out) {
+  if (!adj || maxd < 1 || nodes < 2) return set_error(GG_ERR_INVALID, "bad feature arguments");
+  std::vector<int32_t> own;
+  int32_t *h = hist;
+  if (!h) {
+    own.assign((size_t)nodes * (size_t)(maxd + 1), 0);
+    h = own.data();
+  }
+  gg_graph g = {nodes, degree, nodes, 1, 0, 0, adj, nullptr};
+  gg_result r;
+  const int rc = eval_host(1, &g, &r, false, maxd, h);
+  if (rc) return rc;
+  if (out) *out = r;
+  if (ecc) {
+    for (int v = 0; v < nodes; ++v) {
+      const int32_t *row = h + (size_t)v * (size_t)(maxd + 1);
+      int64_t reached = 0;
+      int far = 0;
+      for (int k = 0; k <= maxd; ++k) {
+        reached += row[k];
+        if (row[k] > 0) far = k;
+      }
+      ecc[v] = reached < nodes ? -1 : far;  // far == maxd: at least maxd
+    }
+  }
+  return GG_OK;
+}
+
 int gg_trace_levels(const gg_graph *g, int units, uint64_t *trace, gg_result *out) {
   if (!g || !trace || !out || units < 1) return set_error(GG_ERR_INVALID, "bad trace arguments");
   g_trace_units = units;

@@ -237,6 +237,20 @@ def set_word_bits(bits: int) -> None:
     L.check(L.lib().gg_set_word_bits(bits))
 
 
+def features(adj, maxd: int):
+    """Per-vertex features of one evaluation: (ecc, hist, result) with
+    ecc[v] = eccentricity of v (maxd when some vertex is at distance >= maxd,
+    -1 when v does not reach every vertex) and hist as distance_histogram."""
+    adj = L.c32(adj)
+    n, d = adj.shape
+    ecc = np.zeros(n, dtype=np.int32)
+    hist = np.zeros((n, maxd + 1), dtype=np.int32)
+    r = L.GGResult()
+    L.check(L.lib().gg_eval_features(n, d, adj.ctypes.data_as(L.i32p), maxd, ecc.ctypes.data_as(L.i32p),
+                                     hist.ctypes.data_as(L.i32p), ctypes.byref(r)))
+    return ecc, hist, _result(r, n)
+
+
 def distance_histogram(adj, maxd: int) -> np.ndarray:
     """hist[v, k] = vertices at distance exactly k from v (k < maxd);
     hist[v, maxd] = vertices at distance >= maxd."""

@@ -132,6 +132,13 @@ int gg_tune_table_save(const char *path);
  * hist[v][k] = vertices at distance exactly k from v (k < maxd), hist[v][maxd]
  * = vertices at distance >= maxd.  Adjacency in host memory. */
 int gg_distance_histogram(int nodes, int degree, const int32_t *adj, int maxd, int32_t *hist);
+/* The same evaluation returning per-vertex features for ranking models (no
+ * reference counterpart): hist as above (may be NULL), ecc[v] = v's
+ * eccentricity (maxd when some vertex is at distance >= maxd, -1 when v does
+ * not reach every vertex; may be NULL) and the evaluation's result in *out
+ * (may be NULL; all n vertices are sources). */
+int gg_eval_features(int nodes, int degree, const int32_t *adj, int maxd, int32_t *ecc,
+                     int32_t *hist, gg_result *out);
 
 /* ---- Profiling aid: one evaluation that records, for the first `units`
  * source tiles, per BFS level k (k < 64) the shader-clock cycles since the

@@ -289,3 +289,30 @@ def test_gpu_distance_histogram():
     got = E.distance_histogram(adj, maxd)
     _, want = oracle.distance_histogram(adj, maxd)
     assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("n,d,maxd,seed", [(500, 6, 16, 1), (1200, 4, 5, 2), (300, 8, 3, 3)])
+def test_gpu_eval_features(n, d, maxd, seed):
+    """gg_eval_features: the histogram of gg_distance_histogram, eccentricities
+    from per-source BFS (capped at maxd), and the evaluation's result."""
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import shortest_path
+    adj = G.edges_to_adj(n, d, G.random_regular(n, d, seed))
+    ecc, hist, res = E.features(adj, maxd)
+    dist = shortest_path(csr_matrix((np.ones(n * d), (np.repeat(np.arange(n), d), adj.ravel())),
+                                    shape=(n, n)), unweighted=True)
+    assert (ecc == np.minimum(dist.max(axis=1), maxd).astype(np.int32)).all()
+    _, want_hist = oracle.distance_histogram(adj, maxd)
+    assert (hist == want_hist).all()
+    want = oracle.matrix_op(adj)
+    assert (res.diameter, res.sum) == (want.raw_diameter, want.S)
+
+
+def test_gpu_eval_features_disconnected():
+    a = G.edges_to_adj(200, 4, G.random_regular(200, 4, 4))
+    k5 = np.array([[200 + j for j in range(5) if j != i] for i in range(5)], dtype=np.int32)
+    adj = np.concatenate([a, k5]).astype(np.int32)
+    ecc, hist, res = E.features(adj, 32)
+    assert not res.connected
+    assert (ecc == -1).all()  # no vertex reaches both components
+    assert (hist[:200].sum(axis=1) == 200).all() and (hist[200:].sum(axis=1) == 5).all()
