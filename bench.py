@@ -108,10 +108,25 @@ def cpu_baseline(items, budget_s):
         if time.perf_counter() - t0 > budget_s or runs >= 50:
             break
     med = float(np.median(times))
+    cores = int(oracle.threads())
+    one = ""
+    if it.get("relabel") is None:  # one evaluation on a single thread, for scale
+        oracle.set_threads(1)
+        t = time.perf_counter()
+        oracle.matrix_op(adj, it["based_nodes"], it["groups"], 0)
+        one = f"; 1 thread: {time.perf_counter() - t:.4g} s per evaluation"
+        oracle.set_threads(cores)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return dict(value=1.0 / med, unit="evals/s",
-                cores=int(oracle.threads()), kind="port",
+                cores=cores, kind="port",
                 sample=f"{runs} evaluations of the first workload graph (n={n}), median "
-                       f"{med:.4f} s; OpenMP threads={oracle.threads()}")
+                       f"{med:.4f} s; OpenMP threads={cores}{one}"
+                       + (f"; CPU: {model}" if model else ""))
 
 
 def sa_time_to_aspl(budget_s):

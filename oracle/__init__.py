@@ -72,6 +72,8 @@ def lib():
         L.orc_create_adj.restype = ctypes.c_int
         L.orc_rotate_hash.argtypes = [ctypes.c_int] * 4 + [ip]
         L.orc_threads.restype = ctypes.c_int
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_set_threads.restype = None
         _lib = L
     return _lib
 
@@ -163,6 +165,10 @@ def rotate_hash(nodes: int, height: int, width: int, groups: int) -> np.ndarray:
 
 def threads() -> int:
     return lib().orc_threads()
+
+
+def set_threads(k: int) -> None:
+    lib().orc_set_threads(int(k))
 
 
 # ---------------------------------------------------------------------------

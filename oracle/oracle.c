@@ -34,6 +34,15 @@ int orc_threads(void) {
 #endif
 }
 
+/* Sets the OpenMP thread count of later calls (no-op without OpenMP). */
+void orc_set_threads(int k) {
+#ifdef _OPENMP
+  if (k > 0) omp_set_num_threads(k);
+#else
+  (void)k;
+#endif
+}
+
 static inline uint64_t pop64(uint64_t x) { return (uint64_t)__builtin_popcountll(x); }
 
 /*
