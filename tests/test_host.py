@@ -66,6 +66,20 @@ def test_row_order_mode_is_checked():
     assert L.lib().gg_set_row_order(prev) == 2
 
 
+def test_row_order_and_features_reject_bad_arguments_before_device():
+    adj = np.zeros((64, 4), dtype=np.int32)
+    rows = np.zeros((64, 8), dtype=np.int32)
+    cost = np.zeros(2, dtype=np.uint64)
+    lib = L.lib()
+    for n, d, bits in [(1, 4, 16), (64, 4, 12), (64, 40, 16), (70000, 4, 16)]:
+        assert lib.gg_row_order(n, d, L.ptr(adj), bits, L.ptr(rows), cost.ctypes.data_as(L.u64p)) \
+            == L.GG_ERR_INVALID
+        assert lib.gg_last_error()
+    ecc = np.zeros(64, dtype=np.int32)
+    assert lib.gg_eval_features(64, 4, L.ptr(adj), 0, L.ptr(ecc), None, None) == L.GG_ERR_INVALID
+    assert lib.gg_eval_features(1, 4, L.ptr(adj), 4, L.ptr(ecc), None, None) == L.GG_ERR_INVALID
+
+
 def test_relabel_must_be_a_permutation():
     rel = np.arange(16, dtype=np.int32)
     rel[3] = 4
