@@ -87,11 +87,68 @@ def algorithmic_bytes(n, d, B, diameter):
     return diameter * n * (4 * d + 8 * w * (d + 2))
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            return next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        return ""
+
+
+_ALGO_NAMES = {1: "MATRIX_OP", 2: "MATRIX_OP_MEM_SAVING"}
+
+
+def reference_baseline(it, budget_s):
+    """The reference's own OpenMP evaluation() (general/evaluation.c:375-391,
+    compiled from source into oracle/_ref by oracle/Makefile) on a bounded
+    sample, in a child process with a time limit.  Returns None when that
+    build is absent or the child fails; the reason goes to stderr."""
+    import subprocess
+    import tempfile
+
+    import oracle
+
+    if it.get("relabel") is not None or it["groups"] != 1 or not oracle.ref_available("general"):
+        return None
+    adj = it["adj"]
+    n = adj.shape[0]
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "adj.npy")
+        np.save(path, np.ascontiguousarray(adj, dtype=np.int32))
+        try:
+            p = subprocess.run([sys.executable, "-m", "oracle.ref_timing", path, str(budget_s)],
+                               cwd=os.path.dirname(os.path.abspath(__file__)), capture_output=True,
+                               text=True, timeout=budget_s + 180)
+        except subprocess.TimeoutExpired:
+            print("reference CPU baseline: time limit reached", file=sys.stderr)
+            return None
+    lines = [ln for ln in p.stdout.splitlines() if ln.startswith("{")]
+    if p.returncode != 0 or not lines:
+        print(f"reference CPU baseline failed (exit {p.returncode}): {p.stderr[-400:]}",
+              file=sys.stderr)
+        return None
+    r = json.loads(lines[-1])
+    med = float(np.median(r["times"]))
+    model = _cpu_model()
+    return dict(value=1.0 / med, unit="evals/s", cores=int(r["threads"]), kind="reference",
+                sample=f"{len(r['times'])} calls of the reference's evaluation() "
+                       f"(general/evaluation.c, {_ALGO_NAMES.get(r['algo'], r['algo'])}, built from source) on "
+                       f"the first workload graph (n={n}), median {med:.4f} s; OpenMP "
+                       f"threads={r['threads']}; 1 thread: {r['one_thread_s']:.4g} s per evaluation; "
+                       f"result diameter {r['diameter']}, S {r['S']}"
+                       + (f"; CPU: {model}" if model else ""))
+
+
 def cpu_baseline(items, budget_s):
-    """The oracle's C/OpenMP port of matrix_op on a bounded sample."""
+    """The reference's own evaluation() where its build is present (general
+    graphs), else the oracle's C/OpenMP port of matrix_op, on a bounded
+    sample."""
     import oracle
 
     it = items[0]
+    ref = reference_baseline(it, budget_s)
+    if ref is not None:
+        return ref
     adj = it["adj"]
     n = adj.shape[0]
     t0 = time.perf_counter()
@@ -116,12 +173,7 @@ def cpu_baseline(items, budget_s):
         oracle.matrix_op(adj, it["based_nodes"], it["groups"], 0)
         one = f"; 1 thread: {time.perf_counter() - t:.4g} s per evaluation"
         oracle.set_threads(cores)
-    model = ""
-    try:
-        with open("/proc/cpuinfo") as f:
-            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
-    except OSError:
-        pass
+    model = _cpu_model()
     return dict(value=1.0 / med, unit="evals/s",
                 cores=cores, kind="port",
                 sample=f"{runs} evaluations of the first workload graph (n={n}), median "
