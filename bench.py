@@ -10,8 +10,10 @@ Prints ONE JSON line.  `value` is evaluations per second; the roofline block
 prices the tile-BFS kernel with the canonical algorithmic bytes of the
 reference's single-tile bit-parallel BFS,
     bytes = L * n * (4*d + 8*w*(d+2)),  w = ceil(B/64),  L = diameter,
-against MI355X's 8 TB/s HBM peak; the CPU baseline is the C/OpenMP port of the
-reference's matrix_op (oracle/) timed on this host on a bounded sample.
+against MI355X's 8 TB/s HBM peak; the CPU baseline is the reference's own
+OpenMP evaluation() (compiled from its sources into oracle/_ref), or where
+that build is absent the C/OpenMP port of its matrix_op (oracle/), timed on
+this host on a bounded sample.
 For the default workload, `sa_time_to_aspl` adds the SA comparison: plain
 single-graph GPU SA for --sa-budget seconds, then surrogate-ranked SA until it
 reaches the same best ASPL (wall-clock ratio = `speedup`).
