@@ -27,7 +27,7 @@ import time
 
 import numpy as np
 
-from . import Eval, ref_available, ref_evaluation, set_threads, threads  # noqa: F401
+from . import ref_available, ref_evaluation, set_threads, threads
 
 MATRIX_OP, MATRIX_OP_MEM_SAVING = 1, 2
 MATRIX_OP_THRESHOLD = 2147483648  # bytes, general/common.h:61
