@@ -114,21 +114,31 @@ __device__ __forceinline__ void flag_counts(int32_t *flags, int bad, int asym) {
   }
 }
 
-// All graphs of a launch set in one launch, one thread per entry of the
-// padded row-major table (coalesced reads of the int32 rows): blockIdx.y
-// selects the graph.
+// All graphs of a launch set in one launch, one thread per 16-byte chunk (8
+// entries) of the padded row-major table, written with one vector store;
+// blockIdx.y selects the graph.  Invalid and asymmetric entries are rare, so
+// a thread that finds some adds its counts with its own atomics.
 __global__ void __launch_bounds__(256) gg_prep_kernel(const PrepDesc *__restrict__ descs) {
   const PrepDesc P = descs[blockIdx.y];  // descs: this launch's first graph
+  const uint32_t nch = (uint32_t)P.dp >> 3;
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool live = k < (uint32_t)P.n * (uint32_t)P.dp;
+  if (k >= (uint32_t)P.n * nch) return;
+  const int i = (int)(k / nch), j0 = 8 * (int)(k - (uint32_t)i * nch);
+  const int ri = P.relabel ? P.relabel[i] : i;
   int bad = 0, asym = 0;
-  if (live) {
-    const int i = (int)(k / (uint32_t)P.dp), j = (int)(k - (uint32_t)i * (uint32_t)P.dp);
-    const int ri = P.relabel ? P.relabel[i] : i;
-    P.adjR[(size_t)ri * P.dp + j] =
-        (uint16_t)prep_entry(P.adj, P.relabel, P.n, P.d, i, j, ri, P.check_sym, &bad, &asym);
+  uint32_t w[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    int b0 = 0, a0 = 0, b1 = 0, a1 = 0;
+    const uint32_t lo = prep_entry(P.adj, P.relabel, P.n, P.d, i, j0 + 2 * q, ri, P.check_sym, &b0, &a0);
+    const uint32_t hi = prep_entry(P.adj, P.relabel, P.n, P.d, i, j0 + 2 * q + 1, ri, P.check_sym, &b1, &a1);
+    w[q] = (lo & 0xffffu) | (hi << 16);
+    bad += b0 + b1;
+    asym += a0 + a1;
   }
-  if (__builtin_amdgcn_ballot_w64(live) != 0ull) flag_counts(P.flags, bad, asym);
+  *(Chunk *)(P.adjR + (size_t)ri * P.dp + j0) = Chunk{w[0], w[1], w[2], w[3]};
+  if (bad) atomicAdd(&P.flags[0], bad);
+  if (asym) atomicAdd(&P.flags[1], asym);
 }
 
 // Patched variant for batches of proposals that differ from one base graph
@@ -1034,10 +1044,10 @@ int enqueue(Assembly &A, DevSet &D, const int32_t *const *adj_src, const int32_t
     (void)adj_src;
     (void)rel_src;
     (void)check_sym;
-    uint32_t max_entries = 1;
-    for (const Slot &s : A.slots) max_entries = std::max(max_entries, (uint32_t)s.n * (uint32_t)s.dp);
+    uint32_t max_chunks = 1;  // one prep thread per 16-byte row chunk
+    for (const Slot &s : A.slots) max_chunks = std::max(max_chunks, (uint32_t)s.n * (uint32_t)(s.dp / 8));
     for (int g0 = 0; g0 < count; g0 += 65535)  // gridDim.y is at most 65535
-      hipLaunchKernelGGL(gg_prep_kernel, dim3((max_entries + 255) / 256, std::min(65535, count - g0)),
+      hipLaunchKernelGGL(gg_prep_kernel, dim3((max_chunks + 255) / 256, std::min(65535, count - g0)),
                          dim3(256), 0, E.stream, (const PrepDesc *)D.prep.p + g0);
   }
   if (A.order_nch_mask)
@@ -1300,7 +1310,7 @@ int gg_row_order(int nodes, int degree, const int32_t *adj, int word_bits, int32
       hipMemcpyAsync(desc_d.p, &pd, sizeof pd, hipMemcpyHostToDevice, st) ||
       hipMemcpyAsync((char *)desc_d.p + sizeof(PrepDesc), &od, sizeof od, hipMemcpyHostToDevice, st))
     return done(set_error(GG_ERR_HIP, "row-order upload failed"));
-  hipLaunchKernelGGL(gg_prep_kernel, dim3((unsigned)((entries + 255) / 256), 1), dim3(256), 0, st,
+  hipLaunchKernelGGL(gg_prep_kernel, dim3((unsigned)((entries / 8 + 255) / 256), 1), dim3(256), 0, st,
                      (const PrepDesc *)desc_d.p);
   launch_order((const OrderDesc *)((const char *)desc_d.p + sizeof(PrepDesc)), 1, nodes,
                1 << (dp / 8), st);
