@@ -65,20 +65,14 @@ def build_workload(name):
         return [dict(adj=adj, based_nodes=10000, groups=1, symmetric=True,
                      relabel=np.arange(10000, dtype=np.int32))]
     if name == "c3":
-        sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
-        from cases import c3_shapes, two_opt  # seeded shapes shared with the tests
+        from graphgolf_amd import workloads as WL
 
-        out = []
-        for (n, d, s) in c3_shapes(512, 7):
-            e = two_opt(G.random_regular(n, d, s), n, np.random.default_rng(s), 1)
-            out.append(dict(adj=G.edges_to_adj(n, d, e), based_nodes=n, groups=1, symmetric=True))
-        return out
+        return [dict(adj=a, based_nodes=a.shape[0], groups=1, symmetric=True)
+                for a in WL.c3_batch(512, 7)]
     if name == "c4b":
-        sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
-        from cases import grid_two_opt_variants
+        from graphgolf_amd import workloads as WL
 
-        e0 = G.random_grid(100, 100, 4, 2, 1)
-        variants = grid_two_opt_variants(e0, 100, 2, np.random.default_rng(4), 256)
+        _, variants = WL.grid_batch(100, 100, 4, 2, 1, 256, 4)
         return [dict(adj=G.edges_to_adj(10000, 4, e), based_nodes=10000, groups=1, symmetric=True)
                 for e in variants]
     raise SystemExit(f"unknown workload {name}")

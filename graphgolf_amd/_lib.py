@@ -24,6 +24,7 @@ GG_ERR_TOO_LARGE = -2
 GG_ERR_HIP = -3
 GG_ERR_STATE = -4
 GG_GRAPH_SYMMETRIC = 1
+GG_GRAPH_BFS = 2  # the reference's bfs() semantics (include/gg_eval.h)
 
 i32p = ctypes.POINTER(ctypes.c_int32)
 u64p = ctypes.POINTER(ctypes.c_uint64)

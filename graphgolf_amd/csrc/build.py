@@ -8,7 +8,6 @@ Outputs, next to the package so they travel with the tree:
     graphgolf_amd/libgg_grid.so    reference ABI shim: grid evaluation()     (include/gg_grid.h)
     graphgolf_amd/libgg_nonregular_{general,grid}.so  non-regular variants' evaluation()
                                    (include/gg_nonregular.h)
-    graphgolf_amd/gg_lds_probe     LDS random-gather throughput probe (executable)
 """
 from __future__ import annotations
 
@@ -33,7 +32,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-I", INC, 
 HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-I", INC, "-I", HERE, "-Wall",
               "-march=x86-64-v3"]
 
-HIP_SOURCES = ["gg_eval.hip", "gg_mlp.hip", "gg_bsa.hip", "gg_incr.hip"]
+HIP_SOURCES = ["gg_eval.hip", "gg_wide.hip", "gg_mlp.hip", "gg_bsa.hip", "gg_incr.hip"]
 HOST_SOURCES = ["gg_host.cpp", "gg_sa_sym.cpp"]
 
 
@@ -81,10 +80,6 @@ def build(verbose: bool = False) -> None:
         _run(["g++", *HOST_FLAGS, *defs, "-shared", "-o", out + ".tmp", os.path.join(HERE, shim),
               "-L", PKG, "-lggeval", "-Wl,-rpath,$ORIGIN"])
         os.replace(out + ".tmp", out)
-    # The LDS gather-throughput probe (the tile BFS's roofline on this GPU).
-    probe = os.path.join(PKG, "gg_lds_probe")
-    _run([HIPCC, *HIP_FLAGS, os.path.join(HERE, "gg_lds_probe.hip"), "-o", probe + ".tmp"])
-    os.replace(probe + ".tmp", probe)
 
 
 if __name__ == "__main__":

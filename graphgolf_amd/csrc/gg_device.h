@@ -9,7 +9,20 @@
 struct gg_mlp;
 struct gg_incr;
 
+struct gg_graph;
+struct gg_result;
+
 namespace gg {
+// Selects the device and creates the evaluator's stream (idempotent).
+int init(int device);
+// The LDS tile path: `count` graphs (host or device adjacency pointers),
+// optionally with per-vertex distance histograms.
+int eval_host(int count, const gg_graph *gs, gg_result *out, bool device_ptrs, int hist_maxd,
+              int32_t *hist_host);
+// The HBM-resident path (gg_wide.hip): any n; sem 0 = matrix_op, 1 = the
+// reference's bfs() semantics (out->sum = its weighted sum over pairs s < i).
+int wide_eval(const gg_graph &g, int sem, bool device_ptrs, gg_result *out);
+int wide_release();
 // The evaluator's stream (adjacency uploads, plans, histograms run on it).
 hipStream_t engine_stream();
 // Launch shape of one tile-BFS launch of `bits`-wide words over an n-vertex

@@ -66,8 +66,10 @@ struct GraphOut {
 // row-major uint16 [n][dp], dp = d rounded up to 8 so that a row is a whole
 // number of 16-byte chunks (8 neighbour ids each).  Padding entries are the
 // row's own id (a self-loop adds nothing to a reach set).  Invalid entries
-// are replaced by the row's own id (memory-safe) and counted in flags[0];
-// flags[1] counts asymmetric entries when check_sym is set.
+// (neighbour ids or relabel entries outside [0,n)) are replaced by the row's
+// own id, or their row is not stored, and counted in flags[0]; no store
+// leaves the table.  flags[1] counts asymmetric entries when check_sym is
+// set.
 
 struct PrepDesc {  // one graph of a prep launch
   const int32_t *adj, *relabel;
@@ -99,7 +101,12 @@ __device__ __forceinline__ uint32_t prep_entry(const int32_t *__restrict__ adj,
     }
     *asym = (c1 != c2);
   }
-  return (uint32_t)(relabel ? relabel[u] : u);
+  const int ru = relabel ? relabel[u] : u;
+  if ((unsigned)ru >= (unsigned)n) {  // a relabel entry out of range
+    *bad = 1;
+    return (uint32_t)ri;
+  }
+  return (uint32_t)ru;
 }
 
 // Adds this wave's counts to a graph's flags with one atomic per counter.
@@ -125,6 +132,10 @@ __global__ void __launch_bounds__(256) gg_prep_kernel(const PrepDesc *__restrict
   if (k >= (uint32_t)P.n * nch) return;
   const int i = (int)(k / nch), j0 = 8 * (int)(k - (uint32_t)i * nch);
   const int ri = P.relabel ? P.relabel[i] : i;
+  if ((unsigned)ri >= (unsigned)P.n) {  // a relabel entry out of range: no store
+    atomicAdd(&P.flags[0], 1);
+    return;
+  }
   int bad = 0, asym = 0;
   uint32_t w[4];
 #pragma unroll
@@ -157,8 +168,11 @@ gg_prep_patched_kernel(const int32_t *__restrict__ base, const int32_t *__restri
   if (live) {
     const int i = (int)(k / (uint32_t)dp), j = (int)(k - (uint32_t)i * (uint32_t)dp);
     const int ri = relabel ? relabel[i] : i;
-    adjR0[slot * slot_stride + (size_t)ri * dp + j] =
-        (uint16_t)prep_entry(base, relabel, n, d, i, j, ri, 0, &bad, &asym);
+    if ((unsigned)ri < (unsigned)n)
+      adjR0[slot * slot_stride + (size_t)ri * dp + j] =
+          (uint16_t)prep_entry(base, relabel, n, d, i, j, ri, 0, &bad, &asym);
+    else
+      bad = 1;
   }
   if (__builtin_amdgcn_ballot_w64(live) != 0ull) flag_counts(flags0 + 2 * slot, bad, asym);
 }
@@ -174,9 +188,17 @@ gg_patch_kernel(const int32_t *__restrict__ patches, int max_patch, int count,
   if (at < 0 || at >= n * d) return;  // unused entry
   const int i = at / d, j = at - i * d;
   const int ri = relabel ? relabel[i] : i;
+  if ((unsigned)ri >= (unsigned)n) {
+    atomicAdd(&flags0[2 * slot], 1);
+    return;
+  }
   int rv = ri;
   if (val < 0 || val >= n) atomicAdd(&flags0[2 * slot], 1);
   else rv = relabel ? relabel[val] : val;
+  if ((unsigned)rv >= (unsigned)n) {
+    atomicAdd(&flags0[2 * slot], 1);
+    rv = ri;
+  }
   adjR0[slot * slot_stride + (size_t)ri * dp + j] = (uint16_t)rv;
 }
 
@@ -765,7 +787,7 @@ struct Assembly {
   int order_nch_mask = 0, order_max_n = 0;  // set by upload_order
 };
 
-int validate(const gg_graph &g) {
+int validate(const gg_graph &g, bool any_size = false) {
   if (g.nodes < 2 || g.degree < 1 || g.based_nodes < 1 || g.groups < 1 || g.added_centers < 0)
     return set_error(GG_ERR_INVALID, "invalid graph sizes (n=%d d=%d B=%d g=%d c=%d)", g.nodes,
                      g.degree, g.based_nodes, g.groups, g.added_centers);
@@ -774,9 +796,13 @@ int validate(const gg_graph &g) {
                      g.based_nodes, g.groups, g.added_centers, g.nodes);
   if (g.added_centers > g.based_nodes)
     return set_error(GG_ERR_INVALID, "added_centers > based_nodes");
-  if (g.nodes > kMaxNodes)
-    return set_error(GG_ERR_TOO_LARGE, "nodes=%d above the device limit %d", g.nodes, kMaxNodes);
+  if (g.nodes > kMaxNodes && !any_size)
+    return set_error(GG_ERR_TOO_LARGE, "nodes=%d above the tile kernels' limit %d", g.nodes,
+                     kMaxNodes);
   if (g.degree > 4096) return set_error(GG_ERR_INVALID, "degree %d too large", g.degree);
+  if ((int64_t)g.nodes * g.degree >= ((int64_t)1 << 32))
+    return set_error(GG_ERR_TOO_LARGE, "nodes*degree = %lld entries (at most 2^32 - 1)",
+                     (long long)g.nodes * g.degree);
   return GG_OK;
 }
 
@@ -1115,6 +1141,16 @@ int eval_host(int count, const gg_graph *gs, gg_result *out, bool device_ptrs, i
       return rc;
   }
   if ((rc = init(-1))) return rc;
+  if (device_ptrs) {
+    // A device relabel table is checked on the host too (n int32 read back):
+    // with duplicates some rows of the converted table would not be written.
+    for (int i = 0; i < count; ++i) {
+      if (!gs[i].relabel) continue;
+      std::vector<int32_t> h((size_t)gs[i].nodes);
+      GG_HIP(hipMemcpy(h.data(), gs[i].relabel, h.size() * 4, hipMemcpyDeviceToHost));
+      if ((rc = check_permutation(h.data(), gs[i].nodes))) return rc;
+    }
+  }
   Assembly A;
   if ((rc = assemble(count, gs, hist_maxd, A))) return rc;
   DevSet &D = g_single;
@@ -1197,6 +1233,7 @@ extern "C" {
 int gg_init(int device) { return gg::init(device); }
 void gg_finalize(void) {
   g_single.release();
+  gg::wide_release();
   gg::finalize();
 }
 const char *gg_last_error(void) { return gg::last_error(); }
@@ -1206,15 +1243,28 @@ int gg_device_count(void) {
   return c;
 }
 
-int gg_apsp(const gg_graph *g, gg_result *out) {
+// Graphs above the tile kernels' limit, and the bfs() semantics, go to the
+// HBM-resident kernels (gg_wide.hip).
+static int apsp_any(const gg_graph *g, gg_result *out, bool device_ptrs) {
   if (!g || !out || !g->adj) return set_error(GG_ERR_INVALID, "null argument");
-  return eval_host(1, g, out, false, 0, nullptr);
+  const bool bfs = (g->flags & GG_GRAPH_BFS) != 0;
+  if (!bfs && g->nodes <= kMaxNodes) return eval_host(1, g, out, device_ptrs, 0, nullptr);
+  int rc = validate(*g, true);
+  if (rc) return rc;
+  if (bfs && g->relabel) return set_error(GG_ERR_INVALID, "GG_GRAPH_BFS takes no relabel");
+  if (!device_ptrs && g->relabel && (rc = check_permutation(g->relabel, g->nodes))) return rc;
+  if (device_ptrs && g->relabel) {
+    if ((rc = init(-1))) return rc;
+    std::vector<int32_t> h((size_t)g->nodes);
+    GG_HIP(hipMemcpy(h.data(), g->relabel, h.size() * 4, hipMemcpyDeviceToHost));
+    if ((rc = check_permutation(h.data(), g->nodes))) return rc;
+  }
+  return wide_eval(*g, bfs ? 1 : 0, device_ptrs, out);
 }
 
-int gg_apsp_device(const gg_graph *g, gg_result *out) {
-  if (!g || !out || !g->adj) return set_error(GG_ERR_INVALID, "null argument");
-  return eval_host(1, g, out, true, 0, nullptr);
-}
+int gg_apsp(const gg_graph *g, gg_result *out) { return apsp_any(g, out, false); }
+
+int gg_apsp_device(const gg_graph *g, gg_result *out) { return apsp_any(g, out, true); }
 
 int gg_eval_batch(int count, const gg_graph *graphs, gg_result *out) {
   if (count <= 0 || !graphs || !out) return set_error(GG_ERR_INVALID, "bad batch arguments");

@@ -13,6 +13,7 @@
 extern "C" __attribute__((weak)) void timer_start(const int) {}
 extern "C" __attribute__((weak)) void timer_stop(const int) {}
 static const int kTimerApsp = 2;  // TIMER_APSP, general/common.h:41
+static const int kAlgoBfs = 0;    // BFS, general/common.h:58
 
 // Fatal errors end the process, as the reference's ERROR() does
 // (general/common.h:67-70).
@@ -32,18 +33,33 @@ extern "C" bool evaluation(const int nodes, const int based_nodes, const int gro
                            const int degree, int *adj, int *diameter, double *ASPL,
                            const int added_centers, const int algo) {
   (void)lines;
-  (void)algo;  // BFS / MATRIX_OP / MEM_SAVING: one device path (see gg_general.h)
   timer_start(kTimerApsp);
-  gg_graph g = {nodes, degree, based_nodes, groups, added_centers, 0, adj, nullptr};
+  const bool bfs = algo == kAlgoBfs;
+  gg_graph g = {nodes, degree, based_nodes, groups, added_centers, bfs ? GG_GRAPH_BFS : 0, adj,
+                nullptr};
   gg_result r;
   if (gg_apsp(&g, &r) != GG_OK) fatal("evaluation");
-  bool ok = r.connected && r.diameter < nodes;  // general/evaluation.c:254
-  if (ok) {
-    *diameter = r.diameter;
-    *ASPL = (double)r.sum / (((double)nodes - 1) * nodes);
+  bool ok;
+  if (bfs) {
+    // general/evaluation.c:137-147: "reached" covers the targets i > s.
+    ok = r.connected != 0;
+    if (ok) {
+      *diameter = r.diameter;
+      *ASPL = (double)r.sum / ((((double)nodes - 1) * nodes) / 2);
+    } else {
+      *diameter = INT_MAX;
+      *ASPL = DBL_MAX;
+      timer_stop(kTimerApsp);  // bfs() stops the timer here too (:146), then evaluation() again
+    }
   } else {
-    *diameter = INT_MAX;
-    *ASPL = DBL_MAX;
+    ok = r.connected && r.diameter < nodes;  // general/evaluation.c:254
+    if (ok) {
+      *diameter = r.diameter;
+      *ASPL = (double)r.sum / (((double)nodes - 1) * nodes);
+    } else {
+      *diameter = INT_MAX;
+      *ASPL = DBL_MAX;
+    }
   }
   timer_stop(kTimerApsp);
   return ok;

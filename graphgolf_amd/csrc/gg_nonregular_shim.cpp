@@ -67,19 +67,21 @@ extern "C" bool evaluation(const int nodes, int based_nodes, const int groups, c
                            const int degree, int *adj, int *diameter, double *ASPL,
                            const int added_centers, const int *num_degrees, const int algo) {
   (void)lines;
-  (void)algo;  // BFS / MATRIX_OP / MEM_SAVING: one device path
   timer_start(kTimerApsp);
-  gg_graph g = {nodes, degree, based_nodes, groups, added_centers, 0,
+  const bool bfs = algo == 0;  // BFS: the variant's bfs() (evaluation.c:57-150)
+  gg_graph g = {nodes, degree, based_nodes, groups, added_centers, bfs ? GG_GRAPH_BFS : 0,
                 padded(nodes, degree, adj, num_degrees), nullptr};
   gg_result r;
   if (gg_apsp(&g, &r) != GG_OK) fatal("evaluation");
-  const bool ok = r.connected && r.diameter < nodes;
+  const bool ok = bfs ? r.connected != 0 : (r.connected && r.diameter < nodes);
   if (ok) {
     *diameter = r.diameter;
-    *ASPL = (double)r.sum / (((double)nodes - 1) * nodes);
+    *ASPL = bfs ? (double)r.sum / ((((double)nodes - 1) * nodes) / 2)
+                : (double)r.sum / (((double)nodes - 1) * nodes);
   } else {
     *diameter = INT_MAX;
     *ASPL = DBL_MAX;
+    if (bfs) timer_stop(kTimerApsp);  // the variant's bfs() stops the timer here too
   }
   timer_stop(kTimerApsp);
   return ok;

@@ -39,14 +39,15 @@ def init(device: int = -1) -> None:
     L.check(L.lib().gg_init(device))
 
 
-def _graph(adj: np.ndarray, based_nodes, groups, added_centers, relabel, symmetric=False):
+def _graph(adj: np.ndarray, based_nodes, groups, added_centers, relabel, symmetric=False,
+           bfs=False):
     n, d = adj.shape
     g = L.GGGraph()
     g.nodes, g.degree = n, d
     g.groups = groups
     g.added_centers = added_centers
     g.based_nodes = (n - added_centers) // groups if based_nodes is None else based_nodes
-    g.flags = L.GG_GRAPH_SYMMETRIC if symmetric else 0
+    g.flags = (L.GG_GRAPH_SYMMETRIC if symmetric else 0) | (L.GG_GRAPH_BFS if bfs else 0)
     g.adj = L.ptr(adj)
     g.relabel = L.ptr(relabel) if relabel is not None else ctypes.cast(None, L.i32p)
     return g
@@ -58,11 +59,13 @@ def _result(r: L.GGResult, n: int) -> Result:
 
 
 def apsp(adj, based_nodes: Optional[int] = None, groups: int = 1, added_centers: int = 0,
-         relabel=None, symmetric: bool = False) -> Result:
-    """One evaluation on the GPU (adjacency from host memory)."""
+         relabel=None, symmetric: bool = False, bfs: bool = False) -> Result:
+    """One evaluation on the GPU (adjacency from host memory).  bfs=True
+    scores with the reference's bfs() (general/evaluation.c:55-150): `sum` is
+    then its weighted sum over pairs s < i (ASPL = sum / (n(n-1)/2))."""
     adj = L.c32(adj)
     rel = L.c32(relabel) if relabel is not None else None
-    g = _graph(adj, based_nodes, groups, added_centers, rel, symmetric)
+    g = _graph(adj, based_nodes, groups, added_centers, rel, symmetric, bfs)
     r = L.GGResult()
     L.check(L.lib().gg_apsp(ctypes.byref(g), ctypes.byref(r)))
     return _result(r, adj.shape[0])
@@ -266,8 +269,15 @@ def distance_histogram(adj, maxd: int) -> np.ndarray:
 def evaluation(nodes: int, based_nodes: int, groups: int, lines: int, degree: int, adj,
                added_centers: int = 0, algo: int = MATRIX_OP):
     """general/evaluation.c:375-391 -> (connected, diameter, ASPL); a
-    disconnected graph gives (False, INT_MAX, DBL_MAX)."""
+    disconnected graph gives (False, INT_MAX, DBL_MAX).  algo = BFS scores
+    with bfs() (general/evaluation.c:55-150), MATRIX_OP and
+    MATRIX_OP_MEM_SAVING with matrix_op (:153-373)."""
     a = L.c32(adj).reshape(nodes, degree)
+    if algo == BFS:
+        r = apsp(a, based_nodes, groups, added_centers, bfs=True)
+        if r.connected:
+            return True, r.diameter, r.sum / (((float(nodes) - 1) * nodes) / 2)
+        return False, INT_MAX, DBL_MAX
     r = apsp(a, based_nodes, groups, added_centers)
     if r.connected and r.diameter < nodes:
         return True, r.diameter, r.sum / ((float(nodes) - 1) * nodes)

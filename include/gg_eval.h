@@ -36,6 +36,12 @@ extern "C" {
 /* Graph flags. */
 #define GG_GRAPH_SYMMETRIC 1  /* caller guarantees adj is symmetric (u in adj[v] <=> v in adj[u],
                                  with multiplicity); skips the device-side check */
+#define GG_GRAPH_BFS 2        /* score with the reference's bfs() (general/evaluation.c:55-150,
+                                 algo = BFS): distances along adj from each source s to the
+                                 targets i > s, weighted (groups - i/based_nodes), centre targets
+                                 x groups, centre sources x 1; "connected" looks at i > s only;
+                                 result.sum is that weighted sum and ASPL = sum / (n(n-1)/2).
+                                 gg_apsp / gg_apsp_device only; relabel must be NULL */
 
 /* One graph to score.  `adj` is row-major int32 [nodes][degree].  `relabel`
  * (may be NULL) is a permutation of 0..nodes-1 applied to every vertex id
@@ -74,11 +80,16 @@ int gg_device_count(void);
 
 /* One evaluation, adjacency in host memory (copied to the device each call,
  * like general/evaluation.cu:83).  Replaces the body of evaluation()
- * (general/evaluation.c:375-391). */
+ * (general/evaluation.c:375-391).  Graphs of up to 65536 vertices run the
+ * LDS-resident tile kernels; larger ones (any n with n*degree < 2^32; the
+ * reference's matrix_op_mem_saving range, general/main.c:411-417) run the
+ * HBM-resident kernels of gg_wide.hip, as does GG_GRAPH_BFS unless the
+ * table is symmetric and rotation invariant (then S/2 is that sum). */
 int gg_apsp(const gg_graph *g, gg_result *out);
 
 /* One evaluation whose adjacency (and relabel) pointers are DEVICE pointers
- * (already resident in HBM).  Same contract as gg_apsp. */
+ * (already resident in HBM).  Same contract as gg_apsp; a relabel table is
+ * read back and checked to be a permutation. */
 int gg_apsp_device(const gg_graph *g, gg_result *out);
 
 /* Score `count` graphs of any sizes in one set of launches (host adjacency).

@@ -32,7 +32,8 @@ bool evaluation(const int nodes, const int max_degree, const int *degree, const 
 #else
 /* Replaces variants/not_regular_general_graph/evaluation.c:378-396
  * (declared in that variant's common.h:103).  adj is [nodes][degree]; row v
- * uses its first num_degrees[v] entries. */
+ * uses its first num_degrees[v] entries.  algo as in gg_general.h: 0 = the
+ * variant's bfs() (:57-150), 1/2 = its matrix_op. */
 bool evaluation(const int nodes, int based_nodes, const int groups, const int lines,
                 const int degree, int *adj, int *diameter, double *ASPL, const int added_centers,
                 const int *num_degrees, const int algo);

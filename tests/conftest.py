@@ -1,6 +1,10 @@
 import os
 import sys
 
+# OpenMP threads of the C oracle and of the reference build sleep instead of
+# spinning while they wait: the suite shares its cores with other processes.
+os.environ.setdefault("OMP_WAIT_POLICY", "PASSIVE")
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 for p in (ROOT, os.path.join(ROOT, "tests", "golden")):
     if p not in sys.path:

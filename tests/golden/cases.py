@@ -11,143 +11,13 @@ import os
 import numpy as np
 
 from graphgolf_amd import graphs as G
+from graphgolf_amd.workloads import (  # noqa: F401  (re-exported for the tests)
+    c3_shapes, grid_two_opt, grid_two_opt_variants, two_opt)
 
 DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
 GOLDEN_CSV = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden_eval.csv")
 GOLDEN_SA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden_sa.json")
 GOLDEN_GRID_SA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden_grid_sa.json")
-
-
-def two_opt(edges: np.ndarray, nodes: int, rng: np.random.Generator, moves: int = 1) -> np.ndarray:
-    """Random 2-opt swaps (a,b),(c,d) -> (a,d),(c,b) keeping the graph simple."""
-    e = edges.copy()
-    present = set(map(tuple, np.sort(e, axis=1).tolist()))
-    done = 0
-    while done < moves:
-        i, j = rng.integers(0, len(e), 2)
-        if i == j:
-            continue
-        a, b = e[i]
-        c, d = e[j]
-        if len({a, b, c, d}) < 4:
-            continue
-        n1, n2 = tuple(sorted((a, d))), tuple(sorted((c, b)))
-        if n1 in present or n2 in present:
-            continue
-        present.discard(tuple(sorted((a, b))))
-        present.discard(tuple(sorted((c, d))))
-        present.add(n1)
-        present.add(n2)
-        e[i] = (a, d)
-        e[j] = (c, b)
-        done += 1
-    return e
-
-
-def grid_two_opt(edges: np.ndarray, height: int, length: int, rng: np.random.Generator,
-                 moves: int = 1) -> np.ndarray:
-    """Random 2-opt swaps of a grid graph keeping it simple and every edge
-    within `length` Manhattan steps (vertex v at (v // height, v % height)).
-    The second line is drawn near the first so that most draws are valid."""
-    e = edges.copy()
-    present = set(map(tuple, np.sort(e, axis=1).tolist()))
-    by_vertex = {}
-    for k, (u, v) in enumerate(e.tolist()):
-        by_vertex.setdefault(u, []).append(k)
-        by_vertex.setdefault(v, []).append(k)
-
-    def dist(u, v):
-        return abs(u // height - v // height) + abs(u % height - v % height)
-
-    width = len(by_vertex) // height
-    ball = [(dx, dy) for dx in range(-length, length + 1) for dy in range(-length, length + 1)
-            if 0 < abs(dx) + abs(dy) <= length]
-    done = 0
-    while done < moves:
-        i = int(rng.integers(0, len(e)))
-        a, b = (int(x) for x in e[i])
-        dx, dy = ball[int(rng.integers(0, len(ball)))]
-        cx, cy = a // height + dx, a % height + dy
-        if not (0 <= cx < width and 0 <= cy < height):
-            continue
-        c0 = cx * height + cy
-        j = by_vertex[c0][int(rng.integers(0, len(by_vertex[c0])))]
-        if i == j:
-            continue
-        c, d = (int(x) for x in e[j])
-        if len({a, b, c, d}) < 4:
-            continue
-        n1, n2 = ((a, d), (c, b)) if rng.integers(0, 2) == 0 else ((a, c), (b, d))
-        if dist(*n1) > length or dist(*n2) > length:
-            continue
-        k1, k2 = tuple(sorted(n1)), tuple(sorted(n2))
-        if k1 in present or k2 in present:
-            continue
-        present -= {tuple(sorted((a, b))), tuple(sorted((c, d)))}
-        present |= {k1, k2}
-        for k, old, new in ((i, (a, b), n1), (j, (c, d), n2)):
-            for x in old:
-                by_vertex[x].remove(k)
-            for x in new:
-                by_vertex[x].append(k)
-            e[k] = new
-        done += 1
-    return e
-
-
-def grid_two_opt_variants(edges: np.ndarray, height: int, length: int,
-                          rng: np.random.Generator, count: int) -> list:
-    """`count` copies of a grid graph, each one valid random 2-opt away from it
-    (same move rules as grid_two_opt)."""
-    present = set(map(tuple, np.sort(edges, axis=1).tolist()))
-    by_vertex = {}
-    for k, (u, v) in enumerate(edges.tolist()):
-        by_vertex.setdefault(u, []).append(k)
-        by_vertex.setdefault(v, []).append(k)
-    width = len(by_vertex) // height
-    ball = [(dx, dy) for dx in range(-length, length + 1) for dy in range(-length, length + 1)
-            if 0 < abs(dx) + abs(dy) <= length]
-
-    def dist(u, v):
-        return abs(u // height - v // height) + abs(u % height - v % height)
-
-    out = []
-    while len(out) < count:
-        i = int(rng.integers(0, len(edges)))
-        a, b = (int(x) for x in edges[i])
-        dx, dy = ball[int(rng.integers(0, len(ball)))]
-        cx, cy = a // height + dx, a % height + dy
-        if not (0 <= cx < width and 0 <= cy < height):
-            continue
-        c0 = cx * height + cy
-        j = by_vertex[c0][int(rng.integers(0, len(by_vertex[c0])))]
-        c, d = (int(x) for x in edges[j])
-        if i == j or len({a, b, c, d}) < 4:
-            continue
-        n1, n2 = ((a, d), (c, b)) if rng.integers(0, 2) == 0 else ((a, c), (b, d))
-        if dist(*n1) > length or dist(*n2) > length:
-            continue
-        if tuple(sorted(n1)) in present or tuple(sorted(n2)) in present:
-            continue
-        e = edges.copy()
-        e[i] = n1
-        e[j] = n2
-        out.append(e)
-    return out
-
-
-def c3_shapes(count: int, seed: int):
-    """(nodes, degree, seed) triples of the uneven batch: n ~ U[1024,16384],
-    d in {8,12,16,24,32}, n*d even."""
-    rng = np.random.default_rng(seed)
-    out = []
-    for k in range(count):
-        d = int(rng.choice([8, 12, 16, 24, 32]))
-        n = int(rng.integers(1024, 16385))
-        if (n * d) % 2:
-            n += 1
-        out.append((n, d, 1000 + k))
-    return out
 
 
 def build(case: dict):
@@ -171,12 +41,40 @@ def build(case: dict):
         n, d, g = case["n"], case["d"], case["g"]
         e = G.random_symmetric(n, d, g, case["seed"])
         return G.edges_to_adj(n, d, e), dict(based_nodes=n // g, groups=g, added_centers=0)
+    if kind == "directed":
+        # Random out-neighbour table (not symmetric), with v -> v+1 (mod n)
+        # in column 0 so that every vertex reaches every other; raw=True
+        # leaves it out.
+        n, d = case["n"], case["d"]
+        rng = np.random.default_rng(case["seed"])
+        adj = rng.integers(0, n, size=(n, d)).astype(np.int32)
+        if not case.get("raw"):
+            adj[:, 0] = (np.arange(n) + 1) % n
+        c = case.get("c", 0)
+        g = case.get("g", 1)
+        return adj, dict(based_nodes=(n - c) // g, groups=g, added_centers=c)
+    if kind == "chain":
+        # Forward edges only (v -> v+1 and random short skips): every s
+        # reaches every i > s and no i < s.
+        n, d = case["n"], case["d"]
+        rng = np.random.default_rng(case["seed"])
+        v = np.arange(n)[:, None]
+        adj = np.minimum(n - 1, v + rng.integers(1, 6, size=(n, d))).astype(np.int32)
+        adj[:, 0] = np.minimum(n - 1, np.arange(n) + 1)
+        return adj, dict(based_nodes=n, groups=1, added_centers=0)
     if kind == "disconnected":
         n, d = case["n"], case["d"]
         e1 = G.random_regular(n // 2, d, case["seed"])
         e2 = G.random_regular(n // 2, d, case["seed"] + 1) + n // 2
         e = np.concatenate([e1, e2])
         return G.edges_to_adj(n, d, e), dict(based_nodes=n, groups=1, added_centers=0)
+    if kind == "grid_rr":
+        w, h, d, g = case["w"], case["h"], case["d"], case.get("g", 1)
+        n = w * h
+        e = G.random_regular(n, d, case["seed"])
+        return G.edges_to_adj(n, d, e), dict(groups=g, based_nodes=n // g, height=h,
+                                             based_height=h if g == 1 else h // 2,
+                                             rotate_hash=G.rotate_hash(n, h, w, g))
     if kind in ("grid_file", "grid"):
         if kind == "grid_file":
             e, w, h = G.read_grid_edges(os.path.join(DATA, case["file"]))
@@ -226,7 +124,51 @@ def general_cases():
     cases.append(dict(id="disc:0", kind="disconnected", n=512, d=4, seed=41, algo=0))
     # C5 full size
     cases.append(dict(id="c5:1", kind="regular", n=65536, d=16, seed=5, algo=1))
+    cases.extend(bfs_cases())
+    cases.extend(large_cases())
     return cases
+
+
+def bfs_cases():
+    """algo = BFS (general/evaluation.c:55-150) where its sum differs from
+    matrix_op's: directed tables (with groups and centres), forward-only
+    tables (bfs() connected, matrix_op not), undirected tables that are not
+    rotation invariant; and rotation-invariant ones, where bfs() = S/2."""
+    cases = []
+    dirs = [("dir:n300", dict(n=300, d=4, seed=51)),
+            ("dir:g2", dict(n=1000, d=6, seed=52, g=2)),
+            ("dir:c40", dict(n=1000, d=5, seed=53, c=40)),
+            ("dir:g4c16", dict(n=1000, d=5, seed=54, g=4, c=16)),
+            ("dir:raw", dict(n=300, d=3, seed=57, raw=True)),
+            ("dir:n20000", dict(n=20000, d=4, seed=56))]
+    for name, kw in dirs:
+        for algo in ((0, 1, 2) if name == "dir:g4c16" else (0, 1)):
+            cases.append(dict(id=f"{name}:a{algo}", kind="directed", algo=algo, **kw))
+    for algo in (0, 1):
+        cases.append(dict(id=f"chain:n500:a{algo}", kind="chain", n=500, d=3, seed=55, algo=algo))
+    # undirected, declared groups but not rotation invariant
+    cases.append(dict(id="ctr:g1:a0", kind="regular", n=1000, d=8, seed=31, c=40, algo=0))
+    cases.append(dict(id="ctr:g4:a0", kind="regular", n=1000, d=8, seed=32, c=16, g=4, algo=0))
+    cases.append(dict(id="ctr:big:a0", kind="regular", n=2600, d=10, seed=33, c=104, g=2, algo=0))
+    for g, n, d in ((2, 512, 6), (4, 1000, 8)):
+        for algo in (0, 1):
+            cases.append(dict(id=f"gasym:g{g}:a{algo}", kind="regular", n=n, d=d, seed=60 + g,
+                              g=g, algo=algo))
+    # rotation invariant: bfs() = S/2
+    for g in (2, 3, 5, 8):
+        cases.append(dict(id=f"sym:g{g}:a0", kind="symmetric", n=120 * g, d=6, g=g, seed=20 + g,
+                          algo=0))
+    return cases
+
+
+def large_cases():
+    """Above 65,536 vertices: the reference's matrix_op at n = 131072 (its
+    A/B buffers exactly 2^31 bytes) and matrix_op_mem_saving at n = 131073
+    (general/main.c:411-417 switches there), groups and centres at that size."""
+    return [dict(id="big:131072", kind="regular", n=131072, d=4, seed=61, algo=1),
+            dict(id="big:131073", kind="regular", n=131073, d=4, seed=62, algo=2),
+            dict(id="big:g2", kind="symmetric", n=140000, d=4, g=2, seed=63, algo=1),
+            dict(id="big:ctr", kind="regular", n=100000, d=4, seed=64, c=1000, algo=1)]
 
 
 def grid_cases():
@@ -243,6 +185,12 @@ def grid_cases():
         for bfs in (False, True):
             cases.append(dict(id=f"grid:g{g}:{int(bfs)}", kind="grid", w=12, h=12, d=4, r=3,
                               seed=50 + g, g=g, bfs=bfs))
+    # above 65,536 vertices (260 x 260), relabelled 4-fold; the table is a
+    # random 4-regular graph (the seeded length-limited generator is
+    # quadratic in n), which the grid evaluation scores like any other
+    for g in (1, 4):
+        cases.append(dict(id=f"grid:big:g{g}", kind="grid_rr", w=260, h=260, d=4, seed=70 + g, g=g,
+                          bfs=False))
     return cases
 
 

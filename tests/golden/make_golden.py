@@ -9,7 +9,8 @@ reference's sa() (general/sa.c:413-520) from fixed seeds; and
 golden_programs.json, the output files of the reference's whole programs
 (main.c included) run on start files this build writes.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py            # everything
+    python tests/golden/make_golden.py --append   # only evaluation cases not in the CSV yet
 """
 from __future__ import annotations
 
@@ -121,14 +122,26 @@ def main() -> None:
     if "--programs" in sys.argv:
         programs()
         return
+    # --append: keep the rows already in the CSV and evaluate only new cases.
+    keep = {}
+    if "--append" in sys.argv and os.path.exists(GOLDEN_CSV):
+        with open(GOLDEN_CSV) as f:
+            keep = {r["id"]: (r["id"], r["ok"], r["diameter"], r["aspl_hex"], r["S"])
+                    for r in csv.DictReader(f)}
     rows = []
     for case in general_cases():
+        if case["id"] in keep:
+            rows.append(keep[case["id"]])
+            continue
         adj, kw = build(case)
         r = oracle.ref_evaluation(adj, kw["based_nodes"], kw["groups"], kw["added_centers"],
                                   case["algo"])
         rows.append((case["id"], int(r.ok), r.diameter, float(r.aspl).hex(), r.S))
         print(rows[-1], flush=True)
     for case in grid_cases():
+        if case["id"] in keep:
+            rows.append(keep[case["id"]])
+            continue
         adj, kw = build(case)
         r = oracle.ref_grid_evaluation(adj, kw["groups"], kw["based_nodes"], kw["height"],
                                        kw["based_height"], kw["rotate_hash"], case["bfs"])
@@ -139,6 +152,8 @@ def main() -> None:
         w = csv.writer(f)
         w.writerow(["id", "ok", "diameter", "aspl_hex", "S"])
         w.writerows(rows)
+    if "--append" in sys.argv:
+        return
     # Simulated-annealing trajectories of the reference's sa() (general/sa.c:413).
     sa_rows = []
     for case in sa_cases():

@@ -50,13 +50,33 @@ def _graph(n=16, d=4, B=None, g=1, c=0, adj=None, relabel=None):
 
 
 @pytest.mark.parametrize("kw", [dict(n=1), dict(d=0), dict(B=5), dict(g=0), dict(c=-1),
-                                dict(n=16, B=4, g=4, c=1), dict(n=70000, d=4)])
+                                dict(n=16, B=4, g=4, c=1)])
 def test_invalid_arguments_rejected_before_device(kw):
     gr, keep = _graph(**kw)
     out = L.GGResult()
     assert L.lib().gg_apsp(ctypes.byref(gr), ctypes.byref(out)) == L.GG_ERR_INVALID or \
         L.lib().gg_apsp(ctypes.byref(gr), ctypes.byref(out)) == L.GG_ERR_TOO_LARGE
     assert L.lib().gg_last_error()
+
+
+def test_oversized_and_bfs_relabel_rejected_before_device():
+    """n*d beyond 32-bit entry offsets, and the bfs() semantics with a
+    relabel table, are refused before any device call (the sizes are read,
+    not the tiny buffers)."""
+    a = np.zeros((16, 4), dtype=np.int32)
+    out = L.GGResult()
+    big = L.GGGraph(1 << 29, 8, 1 << 29, 1, 0, 0, L.ptr(a), ctypes.cast(None, L.i32p))
+    assert L.lib().gg_apsp(ctypes.byref(big), ctypes.byref(out)) == L.GG_ERR_TOO_LARGE
+    assert b"entries" in L.lib().gg_last_error()
+    rel = np.arange(16, dtype=np.int32)
+    bfs = L.GGGraph(16, 4, 16, 1, 0, L.GG_GRAPH_BFS, L.ptr(a), L.ptr(rel))
+    assert L.lib().gg_apsp(ctypes.byref(bfs), ctypes.byref(out)) == L.GG_ERR_INVALID
+    assert b"relabel" in L.lib().gg_last_error()
+    # an out-of-range host relabel entry is refused on the host as well
+    rel[5] = 16
+    g = L.GGGraph(16, 4, 16, 1, 0, 0, L.ptr(a), L.ptr(rel))
+    assert L.lib().gg_apsp(ctypes.byref(g), ctypes.byref(out)) == L.GG_ERR_INVALID
+    assert b"permutation" in L.lib().gg_last_error()
 
 
 def test_row_order_mode_is_checked():
