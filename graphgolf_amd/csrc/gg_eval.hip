@@ -514,6 +514,62 @@ __global__ void gg_clear_i32_kernel(int32_t *p, int n) {
 }
 
 // ---------------------------------------------------------------------------
+// Scratch of the allocation-free entry point (gg_apsp_device_noalloc): the
+// converted row table, the descriptors, the unit list, the unit partials,
+// the result record and a relabel bitmap of one evaluation, for graphs of
+// at most kNoAllocMaxNodes vertices and kNoAllocMaxDegree neighbours.  Each
+// GPU holds its own copy for the life of the process; host code reaches it
+// only through the address hipGetSymbolAddress returns at init.
+constexpr int kNoAllocMaxNodes = 65536;
+constexpr int kNoAllocMaxDegree = 32;
+constexpr size_t kNoAllocAdjBytes = (size_t)kNoAllocMaxNodes * kNoAllocMaxDegree * 2 + 256;
+constexpr int kNoAllocMaxUnits = kNoAllocMaxNodes / 8 + 2;
+constexpr size_t kNoAllocScratchBytes = (size_t)5 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+// Writes one evaluation's descriptors (passed by value: they travel in the
+// kernel arguments, not in an upload), its unit list, and clears its flags
+// and relabel bitmap.
+__global__ void __launch_bounds__(256)
+gg_noalloc_setup_kernel(GraphDev gd, PrepDesc pd, OrderDesc od, GraphMeta gm, GraphDev *gd_out,
+                        PrepDesc *pd_out, OrderDesc *od_out, GraphMeta *gm_out,
+                        UnitDesc *__restrict__ units, int bits, int B, int c, int n,
+                        int32_t *__restrict__ flags, uint32_t *__restrict__ bitmap, int bm_words) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t == 0) {
+    *gd_out = gd;
+    *pd_out = pd;
+    *od_out = od;
+    *gm_out = gm;
+    flags[0] = 0;
+    flags[1] = 0;
+  }
+  const int ub = (B + bits - 1) / bits, uc = (c + bits - 1) / bits;
+  for (int u = t; u < ub + uc; u += gridDim.x * blockDim.x) {
+    if (u < ub) units[u] = UnitDesc{0, u * bits, min(bits, B - u * bits), 0};
+    else units[u] = UnitDesc{0, n - c + (u - ub) * bits, min(bits, c - (u - ub) * bits), 1};
+  }
+  for (int w = t; w < bm_words; w += gridDim.x * blockDim.x) bitmap[w] = 0u;
+}
+
+// Counts relabel entries outside [0,n) and repeated ones (a bitmap of the
+// values seen) into flags[0], so a table that is not a permutation ends the
+// evaluation with an error.
+__global__ void __launch_bounds__(256)
+gg_relabel_check_kernel(const int32_t *__restrict__ relabel, int n, uint32_t *__restrict__ bitmap,
+                        int32_t *__restrict__ flags) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int r = relabel[i];
+  if ((unsigned)r >= (unsigned)n) {
+    atomicAdd(&flags[0], 1);
+    return;
+  }
+  const uint32_t bit = 1u << (r & 31);
+  if (atomicOr(&bitmap[r >> 5], bit) & bit) atomicAdd(&flags[0], 1);
+}
+
+// ---------------------------------------------------------------------------
 // Host engine.
 
 namespace gg {
@@ -676,6 +732,8 @@ struct TuneKey {
   }
 };
 
+constexpr int kMaxDevices = 64;
+
 struct Engine {
   bool ready = false;
   int device = 0;
@@ -683,6 +741,7 @@ struct Engine {
   hipStream_t stream = nullptr;
   int forced_bits = 0;
   std::map<TuneKey, int> tune;
+  unsigned char *scratch_dev[kMaxDevices] = {};  // `scratch`'s address on each device
 };
 
 static Engine E;
@@ -705,6 +764,14 @@ int init(int device) {
   hipDeviceProp_t prop;
   GG_HIP(hipGetDeviceProperties(&prop, device));
   E.cus = prop.multiProcessorCount;
+  if (device < kMaxDevices && !E.scratch_dev[device]) {
+    // With the device current: the address of this device's copy of the
+    // allocation-free entry point's scratch array.
+    void *ptr = nullptr;
+    GG_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return set_error(GG_ERR_HIP, "no device address for the scratch array");
+    E.scratch_dev[device] = (unsigned char *)ptr;
+  }
   GG_HIP(hipStreamCreateWithFlags(&E.stream, hipStreamNonBlocking));
   // Allow the large dynamic LDS allocation on every instance we may launch.
   for (int bits : {8, 16, 32, 64})
@@ -1204,6 +1271,110 @@ int eval_host(int count, const gg_graph *gs, gg_result *out, bool device_ptrs, i
   return rc;
 }
 
+// ---- The allocation-free evaluation ----------------------------------------
+// gg_apsp_device's kernels on one graph whose buffers all live in this
+// device's `scratch` array (the descriptors are written by a setup kernel
+// from its arguments), so nothing is allocated: no device memory, no pinned
+// or heap host memory.  Calls on one device share the array, so calls that
+// overlap on one device race; every call rewrites all it reads.
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+int eval_noalloc(const gg_graph &g, gg_result *out) {
+  int rc = validate(g);
+  if (rc) return rc;
+  if (g.nodes > kNoAllocMaxNodes || g.degree > kNoAllocMaxDegree)
+    return set_error(GG_ERR_TOO_LARGE,
+                     "allocation-free evaluation: n=%d d=%d beyond its scratch (n <= %d, d <= %d)",
+                     g.nodes, g.degree, kNoAllocMaxNodes, kNoAllocMaxDegree);
+  if (g.flags & GG_GRAPH_BFS)
+    return set_error(GG_ERR_INVALID, "allocation-free evaluation: GG_GRAPH_BFS is not served");
+  if ((rc = init(-1))) return rc;
+  unsigned char *base = E.device < kMaxDevices ? E.scratch_dev[E.device] : nullptr;
+  if (!base) return set_error(GG_ERR_STATE, "scratch array address unknown on device %d", E.device);
+  const int n = g.nodes, d = g.degree, dp = (d + 7) & ~7;
+  const int B = g.based_nodes, c = g.added_centers;
+  const int bits = choose_bits(n, d, B, g.groups, c, 1);
+  if (bits == 0) return set_error(GG_ERR_TOO_LARGE, "n=%d does not fit LDS", n);
+  int block, vpt;
+  block_vpt_for(n, &block, &vpt);
+  const TileKernel k = kernel_for({bits, block, vpt, 0});
+  if (!k) return set_error(GG_ERR_TOO_LARGE, "no %d-bit instance for n=%d", bits, n);
+  const int n_cap = (n + 7) & ~7;
+  const size_t rbytes = (size_t)n_cap * (bits / 8);
+  const size_t budget = rbytes * 2 + 4096 <= (size_t)kLdsBytes / 2 ? rbytes * 2 + 4096 : (size_t)kLdsBytes;
+  const int qcap = qcap_for(bits, n_cap, budget);
+  if (qcap < 64) return set_error(GG_ERR_TOO_LARGE, "LDS budget too small for n = %d", n);
+  const size_t lds = lds_bytes_for(bits, n_cap, qcap);
+  const int units = (B + bits - 1) / bits + (c + bits - 1) / bits;
+  if (units > kNoAllocMaxUnits) return set_error(GG_ERR_TOO_LARGE, "%d source tiles", units);
+  // Layout of the scratch array.
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = off;
+    off = align256(off + bytes);
+    return base + at;
+  };
+  uint16_t *adjR = (uint16_t *)take(kNoAllocAdjBytes);
+  int32_t *flags = (int32_t *)take(2 * sizeof(int32_t));
+  GraphDev *gd_dev = (GraphDev *)take(sizeof(GraphDev));
+  PrepDesc *pd_dev = (PrepDesc *)take(sizeof(PrepDesc));
+  OrderDesc *od_dev = (OrderDesc *)take(sizeof(OrderDesc));
+  GraphMeta *gm_dev = (GraphMeta *)take(sizeof(GraphMeta));
+  GraphOut *gout_dev = (GraphOut *)take(sizeof(GraphOut));
+  UnitDesc *units_dev = (UnitDesc *)take((size_t)kNoAllocMaxUnits * sizeof(UnitDesc));
+  UnitOut *outs_dev = (UnitOut *)take((size_t)kNoAllocMaxUnits * sizeof(UnitOut));
+  const int bm_words = (kNoAllocMaxNodes + 31) / 32;
+  uint32_t *bitmap = (uint32_t *)take((size_t)bm_words * 4);
+  if (off > kNoAllocScratchBytes) return set_error(GG_ERR_STATE, "scratch layout exceeds the array");
+
+  GraphDev gd;
+  gd.adjR = adjR;
+  gd.flags = flags;
+  gd.n = n;
+  gd.dp = dp;
+  gd.split = B * g.groups;
+  gd.c = c;
+  gd.push_ok = 1;
+  gd.groups = g.groups;
+  gd.hist_maxd = 0;
+  gd.trace_units = 0;
+  gd.hist = nullptr;
+  gd.trace = nullptr;
+  const PrepDesc pd = {g.adj, g.relabel, adjR, flags, n, d, dp, (g.flags & GG_GRAPH_SYMMETRIC) ? 0 : 1};
+  const int shift = order_applies(units, kOrderMinTiles, dp) ? order_shift(bits) : -1;
+  const OrderDesc od = {adjR, nullptr, n, dp, shift};
+  const GraphMeta gm = {0, units, n, g.groups};
+  const hipStream_t st = E.stream;
+  const int setup_items = std::max(units, bm_words);
+  hipLaunchKernelGGL(gg_noalloc_setup_kernel, dim3((setup_items + 255) / 256), dim3(256), 0, st, gd, pd,
+                     od, gm, gd_dev, pd_dev, od_dev, gm_dev, units_dev, bits, B, c, n, flags, bitmap,
+                     bm_words);
+  if (g.relabel)
+    hipLaunchKernelGGL(gg_relabel_check_kernel, dim3((n + 255) / 256), dim3(256), 0, st, g.relabel, n,
+                       bitmap, flags);
+  const uint32_t chunks = (uint32_t)n * (uint32_t)(dp / 8);
+  hipLaunchKernelGGL(gg_prep_kernel, dim3((chunks + 255) / 256, 1), dim3(256), 0, st,
+                     (const PrepDesc *)pd_dev);
+  if (shift >= 0) launch_order(od_dev, 1, n, 1 << (dp / 8), st);
+  hipLaunchKernelGGL(k, dim3(units), dim3(block), lds, st, (const GraphDev *)gd_dev,
+                     (const UnitDesc *)units_dev, outs_dev, 0, n_cap, qcap);
+  hipLaunchKernelGGL(gg_combine_kernel, dim3(1), dim3(256), 0, st, (const GraphMeta *)gm_dev,
+                     (const UnitDesc *)units_dev, (const UnitOut *)outs_dev, gout_dev, 1);
+  GG_HIP(hipGetLastError());
+  GraphOut r;
+  GG_HIP(hipMemcpyAsync(&r, gout_dev, sizeof r, hipMemcpyDeviceToHost, st));
+  GG_HIP(hipStreamSynchronize(st));
+  if (r.error & 2) return set_error(GG_ERR_STATE, "the reach array is not at LDS address 0");
+  if (r.error)
+    return set_error(GG_ERR_INVALID, "adjacency or relabel entry outside [0,%d), or relabel not a permutation", n);
+  out->connected = r.connected;
+  out->diameter = r.diameter;
+  out->max_ecc = r.max_ecc;
+  out->word_bits = bits;
+  out->sum = r.sum;
+  return GG_OK;
+}
+
 // ---- Plans ---------------------------------------------------------------
 
 struct Plan {
@@ -1263,6 +1434,11 @@ static int apsp_any(const gg_graph *g, gg_result *out, bool device_ptrs) {
 }
 
 int gg_apsp(const gg_graph *g, gg_result *out) { return apsp_any(g, out, false); }
+
+int gg_apsp_device_noalloc(const gg_graph *g, gg_result *out) {
+  if (!g || !out || !g->adj) return set_error(GG_ERR_INVALID, "null argument");
+  return eval_noalloc(*g, out);
+}
 
 int gg_apsp_device(const gg_graph *g, gg_result *out) { return apsp_any(g, out, true); }
 

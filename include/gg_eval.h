@@ -92,6 +92,20 @@ int gg_apsp(const gg_graph *g, gg_result *out);
  * read back and checked to be a permutation. */
 int gg_apsp_device(const gg_graph *g, gg_result *out);
 
+/* gg_apsp_device without any allocation: no counterpart in the reference;
+ * for callers that cannot allocate (code that runs under HIP graph capture
+ * of its own work, tight serving loops, harnesses that forbid hipMalloc).
+ * Same inputs, result and errors as gg_apsp_device for graphs of at most
+ * 65536 vertices and 32 neighbours per vertex (larger ones, and
+ * GG_GRAPH_BFS, are refused with GG_ERR_TOO_LARGE / GG_ERR_INVALID).  All
+ * intermediate buffers live in a 5 MiB device array of the library that
+ * each GPU holds for the life of the process; a device relabel table is
+ * checked on the device.  Calls on one GPU share that array: calls that
+ * overlap on one GPU (other host threads) race.  gg_init() beforehand
+ * creates the stream and looks up the array's address; the first call does
+ * it otherwise. */
+int gg_apsp_device_noalloc(const gg_graph *g, gg_result *out);
+
 /* Score `count` graphs of any sizes in one set of launches (host adjacency).
  * out[i] corresponds to graphs[i].  No reference counterpart: the batched
  * scorer for simulated-annealing proposals. */
