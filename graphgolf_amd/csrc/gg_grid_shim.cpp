@@ -16,6 +16,21 @@ static void fatal(const char *what) {
   exit(1);
 }
 
+// The device is brought up with libc random() switched to a private state:
+// the reference's callers seed random() before their first evaluation()
+// (general/main.c:360, grid/main.c:630) and replay its stream, and the HIP
+// runtime's start-up must not draw from it.
+static void init_keeping_random() {
+  static bool done = false;
+  if (done) return;
+  static char own_state[256];
+  char *caller = initstate(1u, own_state, sizeof own_state);
+  const int rc = gg_init(-1);
+  setstate(caller);
+  if (rc != GG_OK) fatal("device initialisation");
+  done = true;
+}
+
 extern "C" bool evaluation(const int nodes, const int degree, const int groups, const int *adjacency,
                            const int based_nodes, const int height, const int based_height,
                            int *diameter, double *ASPL, const bool enable_bfs,
@@ -23,6 +38,7 @@ extern "C" bool evaluation(const int nodes, const int degree, const int groups, 
   (void)height;
   (void)based_height;  // the base quadrant is rotate_hash^-1(0..based_nodes-1)
   timer_start(kTimerApsp);
+  init_keeping_random();
   gg_graph g = {nodes, degree, based_nodes, groups, 0, 0, adjacency, rotate_hash};
   gg_result r;
   if (gg_apsp(&g, &r) != GG_OK) fatal("evaluation");

@@ -21,6 +21,21 @@ static void fatal(const char *what) {
   exit(1);
 }
 
+// The device is brought up with libc random() switched to a private state:
+// the reference's callers seed random() before their first evaluation()
+// (general/main.c:360, grid/main.c:630) and replay its stream, and the HIP
+// runtime's start-up must not draw from it.
+static void init_keeping_random() {
+  static bool done = false;
+  if (done) return;
+  static char own_state[256];
+  char *caller = initstate(1u, own_state, sizeof own_state);
+  const int rc = gg_init(-1);
+  setstate(caller);
+  if (rc != GG_OK) fatal("device initialisation");
+  done = true;
+}
+
 // Row v: its first len[v] entries, then v itself in the unused slots.
 static const int32_t *padded(int nodes, int width, const int *adj, const int *len) {
   static thread_local std::vector<int32_t> buf;
@@ -48,6 +63,7 @@ extern "C" bool evaluation(const int nodes, const int max_degree, const int *deg
   (void)height;
   (void)based_height;
   timer_start(kTimerApsp);
+  init_keeping_random();
   gg_graph g = {nodes, max_degree, based_nodes, groups, 0, 0,
                 padded(nodes, max_degree, adjacency, degree), rotate_hash};
   gg_result r;
@@ -68,6 +84,7 @@ extern "C" bool evaluation(const int nodes, int based_nodes, const int groups, c
                            const int added_centers, const int *num_degrees, const int algo) {
   (void)lines;
   timer_start(kTimerApsp);
+  init_keeping_random();
   const bool bfs = algo == 0;  // BFS: the variant's bfs() (evaluation.c:57-150)
   gg_graph g = {nodes, degree, based_nodes, groups, added_centers, bfs ? GG_GRAPH_BFS : 0,
                 padded(nodes, degree, adj, num_degrees), nullptr};

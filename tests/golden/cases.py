@@ -334,8 +334,13 @@ def program_cases():
         dict(id="gen_multi_N", kind="general", input=("multi", 60, 4, 21), args="-N -s 4 -n 300"),
         dict(id="gen_multi_N_g2", kind="general", input=("multi", 40, 4, 22),
              args="-N -g 2 -s 5 -n 300"),
+        dict(id="gen_n64d4_B", kind="general", input=("regular", 64, 4, 7), args="-B -s 3 -n 200"),
+        dict(id="gen_g4_v4_B", kind="general", input=("regular", 24, 3, 15),
+             args="-B -g 4 -v 4 -s 7 -n 150 -w 20 -c 0.5"),
         dict(id="grid_lat_8x8_d", kind="grid", input=None, args="-W 8 -H 8 -D 4 -R 3 -s 4 -d"),
         dict(id="grid_lat_6x6", kind="grid", input=None, args="-W 6 -H 6 -D 4 -R 3 -s 5 -n 100"),
+        dict(id="grid_lat_6x6_B", kind="grid", input=None,
+             args="-W 6 -H 6 -D 4 -R 3 -s 5 -n 100 -B"),
         dict(id="grid_lat_6x6_g2", kind="grid", input=None,
              args="-W 6 -H 6 -D 4 -R 2 -g 2 -s 5 -n 100"),
         dict(id="grid_lat_8x8_g4", kind="grid", input=None,
