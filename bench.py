@@ -331,6 +331,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0,
                     help="seconds of CPU time for the CPU baseline sample (0 = skip)")
     ap.add_argument("--word-bits", type=int, default=0)
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuner (the built-in configuration runs)")
     ap.add_argument("--sa-budget", type=float, default=20.0,
                     help="seconds of plain GPU SA for the time-to-ASPL comparison (c5 only; 0 = skip)")
     args = ap.parse_args()
@@ -347,8 +349,14 @@ def main():
         return
     items = build_workload(args.workload)
     E.init(0)
+    tuned = None
     if args.word_bits:
         E.set_word_bits(args.word_bits)
+    elif not args.no_tune:
+        # The autotuner times its candidate configurations on the workload's
+        # own graphs and keeps the fastest for their shape bucket; the plan
+        # below runs that choice.
+        tuned, _ = E.autotune_batch(items, reps=3)
     plan = E.Plan([dict(nodes=it["adj"].shape[0], degree=it["adj"].shape[1],
                         based_nodes=it["based_nodes"], groups=it["groups"],
                         relabel=it.get("relabel"), symmetric=it["symmetric"]) for it in items])
@@ -405,7 +413,8 @@ def main():
         "config": {"workload": WORKLOADS[args.workload], "graphs": len(items),
                    "nodes": int(items[0]["adj"].shape[0]), "degree": int(items[0]["adj"].shape[1]),
                    "groups": int(items[0]["groups"]), "word_bits": r0.word_bits,
-                   "diameter": r0.diameter, "sum": r0.sum, "aspl": r0.aspl},
+                   "diameter": r0.diameter, "sum": r0.sum, "aspl": r0.aspl,
+                   "launch": plan.config(), "tuned": tuned is not None},
         "canonical_effective_GBps": round(achieved, 1),
         "roofline": lds_roofline(args.workload, k_ms, traffic, total_bytes),
         "cpu_baseline": cpu,

@@ -124,8 +124,16 @@ def lib():
         "gg_row_order": ([ctypes.c_int, ctypes.c_int, i32p, ctypes.c_int, i32p, u64p], ctypes.c_int),
         "gg_autotune": ([ctypes.c_int] * 6 + [ctypes.POINTER(ctypes.c_int),
                                               ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+        "gg_autotune_batch": ([ctypes.c_int, ctypes.POINTER(GGGraph), ctypes.c_int, i32p,
+                               ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+        "gg_tune_lookup": ([ctypes.c_int] * 6 + [i32p], ctypes.c_int),
+        "gg_plan_config": ([ctypes.c_void_p, i32p], ctypes.c_int),
+        "gg_last_error_code": ([], ctypes.c_int),
+        "gg_apsp_device_noalloc": ([ctypes.POINTER(GGGraph), ctypes.POINTER(GGResult)],
+                                   ctypes.c_int),
         "gg_tune_table_load": ([ctypes.c_char_p], ctypes.c_int),
         "gg_tune_table_save": ([ctypes.c_char_p], ctypes.c_int),
+        "gg_tune_table_clear": ([], ctypes.c_int),
         "gg_sa_general": ([ctypes.c_int, ctypes.c_int, i32p, ctypes.POINTER(GGSaParams),
                            EVAL_FN, ctypes.c_void_p, ctypes.POINTER(GGSaStats), i32p, u64p],
                           ctypes.c_int),

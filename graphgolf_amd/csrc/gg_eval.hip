@@ -622,15 +622,18 @@ gg_relabel_check_kernel(const int32_t *__restrict__ relabel, int n, uint32_t *__
 namespace gg {
 
 static thread_local char g_err[512] = "";
+static thread_local int g_err_code = GG_OK;
 
 int set_error(int code, const char *fmt, ...) {
   va_list ap;
   va_start(ap, fmt);
   vsnprintf(g_err, sizeof g_err, fmt, ap);
   va_end(ap);
+  g_err_code = code;
   return code;
 }
 const char *last_error() { return g_err; }
+int last_error_code() { return g_err_code; }
 
 #define GG_HIP(call)                                                                  \
   do {                                                                                \
@@ -739,8 +742,53 @@ static TileKernel kernel_for(const KernelKey &k) {
   return nullptr;
 }
 
-static void block_vpt_for(int n, int *block, int *vpt) {
-  if (n <= 2048) {
+// The persistent (queue-driven) instances, kTileNormal only.
+typedef void (*QueueKernel)(const GraphDev *, const UnitDesc *, UnitOut *, int, int, int, int,
+                            int *);
+
+template <typename W, int B, int V>
+static QueueKernel qptr() {
+  return (QueueKernel)&gg_tile_queue_kernel<W, B, V, kTileNormal>;
+}
+
+template <typename W>
+static QueueKernel pick_qvpt(int block, int vpt) {
+  if (block == 256) {
+    switch (vpt) {
+      case 1: return qptr<W, 256, 1>();
+      case 2: return qptr<W, 256, 2>();
+      case 4: return qptr<W, 256, 4>();
+      case 8: return qptr<W, 256, 8>();
+    }
+  } else {
+    switch (vpt) {
+      case 4: return qptr<W, 1024, 4>();
+      case 8: return qptr<W, 1024, 8>();
+      case 16: return qptr<W, 1024, 16>();
+      case 32: return qptr<W, 1024, 32>();
+      case 64:
+        if constexpr (sizeof(W) <= 4) return qptr<W, 1024, 64>();
+        break;
+    }
+  }
+  return nullptr;
+}
+
+static QueueKernel queue_kernel_for(const KernelKey &k) {
+  if (k.hist) return nullptr;
+  switch (k.bits) {
+    case 8: return pick_qvpt<uint8_t>(k.block, k.vpt);
+    case 16: return pick_qvpt<uint16_t>(k.block, k.vpt);
+    case 32: return pick_qvpt<uint32_t>(k.block, k.vpt);
+    case 64: return pick_qvpt<uint64_t>(k.block, k.vpt);
+  }
+  return nullptr;
+}
+
+// Threads per block and vertices per thread for an n-vertex graph: blocks
+// of 256 up to 2048 vertices unless `want_block` asks for 1024.
+static void block_vpt_pref(int n, int want_block, int *block, int *vpt) {
+  if (n <= 2048 && want_block != 1024) {
     *block = 256;
     int v = (n + 255) / 256;
     *vpt = v <= 1 ? 1 : v <= 2 ? 2 : v <= 4 ? 4 : 8;
@@ -751,13 +799,13 @@ static void block_vpt_for(int n, int *block, int *vpt) {
   }
 }
 
+static void block_vpt_for(int n, int *block, int *vpt) { block_vpt_pref(n, 0, block, vpt); }
+
+
 static size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
 
 static size_t lds_bytes_for(int bits, int n_cap, int qcap) {
-  size_t wb = bits / 8;
-  // red[2][16][4] | qcount[2] (+2 pad) | sacc[2][64] (kTileSrc reached counts)
-  return align16((size_t)n_cap * wb) + align16((size_t)2 * qcap * wb) +
-         align16((size_t)2 * qcap * 2) + 2 * 16 * 4 * 4 + 16 + 2 * 64 * 4;
+  return tile_lds_bytes((size_t)bits / 8, n_cap, qcap);
 }
 
 // The largest queue that fits the LDS budget, capped at what push levels
@@ -771,13 +819,27 @@ static int qcap_for(int bits, int n, size_t budget) {
   return (int)std::min(fit, (size_t)want);
 }
 
-// Tuning table: (n bucket, d) -> word bits.
+// Launch configuration of one graph, or of a whole batch: reach-word width,
+// threads per block (0: from n), row-order mode (-1: the global mode) and,
+// for batches, whether one queue-driven launch drains every graph.
+struct Config {
+  int bits, block, order, persistent;
+};
+
+// Tuning table key: a shape bucket.  nb = ceil(log2 n) (of the largest graph
+// for a batch), degree, groups, whether there are centres, batch or single.
 struct TuneKey {
-  int n, d, b, g, batch;
+  int nb, d, g, c, batched;
   bool operator<(const TuneKey &o) const {
-    return std::tie(n, d, b, g, batch) < std::tie(o.n, o.d, o.b, o.g, o.batch);
+    return std::tie(nb, d, g, c, batched) < std::tie(o.nb, o.d, o.g, o.c, o.batched);
   }
 };
+
+static int bucket_of(int n) {
+  int b = 0;
+  while ((1 << b) < n) ++b;
+  return b;
+}
 
 constexpr int kMaxDevices = 64;
 
@@ -787,7 +849,9 @@ struct Engine {
   int cus = 256;
   hipStream_t stream = nullptr;
   int forced_bits = 0;
-  std::map<TuneKey, int> tune;
+  std::map<TuneKey, Config> tune;
+  bool forced_cfg = false;  // the tuner's candidate runs
+  Config forced{0, 0, -1, 0};
   unsigned char *scratch_dev[kMaxDevices] = {};  // `scratch`'s address on each device
 };
 
@@ -829,6 +893,9 @@ int init(int device) {
           if (!k) continue;
           GG_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      kLdsBytes));
+          if (QueueKernel q = queue_kernel_for({bits, block, vpt, h}))
+            GG_HIP(hipFuncSetAttribute((const void *)q, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       kLdsBytes));
         }
   E.device = device;
   E.ready = true;
@@ -864,17 +931,44 @@ int max_bits_for(int n) {
 // Word width for one graph.  Single evaluations favour narrow words when the
 // graph has few source tiles (more workgroups in flight); batches favour the
 // widest word (fewest adjacency passes).
+// The launch configuration of one graph (batched = false) or of a batch
+// whose largest graph has this shape: the tuning table's entry for the
+// shape bucket when there is one, else the built-in choice -- single
+// graphs narrow the word until the graph has >= CUs/2 source tiles (more
+// workgroups in flight), batches take the widest word that fits LDS and one
+// queue-driven launch.  gg_set_word_bits overrides the width.
+Config config_for(int n, int d, int b, int g, int c, bool batched) {
+  Config cfg{0, 0, -1, batched ? 1 : 0};
+  const int maxb = max_bits_for(n);
+  if (maxb == 0) return cfg;
+  if (batched) {
+    cfg.bits = maxb;
+  } else {
+    int bits = maxb;
+    const int srcs = b + c;
+    while (bits > 8 && (srcs + bits - 1) / bits < E.cus / 2) bits /= 2;
+    cfg.bits = bits;
+  }
+  auto it = E.tune.find({bucket_of(n), d, g, c > 0 ? 1 : 0, batched ? 1 : 0});
+  if (it != E.tune.end()) cfg = it->second;
+  if (E.forced_cfg) cfg = E.forced;
+  if (E.forced_bits) cfg.bits = E.forced_bits;
+  if (cfg.bits < 8 || cfg.bits > maxb) cfg.bits = maxb;
+  if (cfg.block == 256 && n > 2048) cfg.block = 0;
+  if (!batched) cfg.persistent = 0;
+  return cfg;
+}
+
 int choose_bits(int n, int d, int b, int g, int c, int batch) {
-  int maxb = max_bits_for(n);
-  if (maxb == 0) return 0;
-  if (E.forced_bits) return std::min(E.forced_bits, maxb);
-  auto it = E.tune.find({n, d, b, g, batch > 1 ? 1 : 0});
-  if (it != E.tune.end() && it->second <= maxb) return it->second;
-  if (batch > 1) return maxb;
-  int bits = maxb;
-  const int srcs = b + c;
-  while (bits > 8 && (srcs + bits - 1) / bits < E.cus / 2) bits /= 2;
-  return bits;
+  return config_for(n, d, b, g, c, batch > 1).bits;
+}
+
+// The row-order mode a configuration runs with: an explicit gg_set_row_order
+// (0 or 2) wins over the table's choice; the default mode (1) takes it.
+extern int g_row_order;
+int effective_order(const Config &cfg) {
+  if (g_row_order != 1 || cfg.order < 0) return g_row_order;
+  return cfg.order;
 }
 
 // ---- Batch assembly -------------------------------------------------------
@@ -889,6 +983,7 @@ struct LaunchGroup {
   KernelKey key;
   int n_cap, qcap, unit_begin, unit_count;
   size_t lds;
+  int grid;  // persistent launches: workgroups (resident per CU x CUs, at most the units)
 };
 
 struct Assembly {
@@ -899,6 +994,9 @@ struct Assembly {
   std::vector<LaunchGroup> groups;
   size_t adj_elems = 0, adjT_elems = 0, rel_elems = 0;
   int order_nch_mask = 0, order_max_n = 0;  // set by upload_order
+  bool persistent = false;  // one queue-driven launch for every graph
+  int order_mode = 1;       // row-order mode of this set
+  Config cfg{0, 0, -1, 0};  // the configuration it was assembled with
 };
 
 int validate(const gg_graph &g, bool any_size = false) {
@@ -923,10 +1021,22 @@ int validate(const gg_graph &g, bool any_size = false) {
 int assemble(int count, const gg_graph *gs, int hist_maxd, Assembly &A) {
   A = Assembly();
   A.slots.resize(count);
+  const bool batched = count > 1;
+  // A batch is configured by its largest graph's shape bucket.
+  int big = 0;
+  for (int i = 1; i < count; ++i)
+    if (gs[i].nodes > gs[big].nodes) big = i;
+  int rc = validate(gs[big]);
+  if (rc) return rc;
+  const Config bc = config_for(gs[big].nodes, gs[big].degree, gs[big].based_nodes, gs[big].groups,
+                               gs[big].added_centers, batched);
+  A.cfg = bc;
+  A.persistent = batched && bc.persistent && hist_maxd == 0;
+  A.order_mode = effective_order(bc);
+  const int n_max = gs[big].nodes;
   for (int i = 0; i < count; ++i) {
     const gg_graph &g = gs[i];
-    int rc = validate(g);
-    if (rc) return rc;
+    if ((rc = validate(g))) return rc;
     Slot &s = A.slots[i];
     s.n = g.nodes;
     s.d = g.degree;
@@ -935,9 +1045,11 @@ int assemble(int count, const gg_graph *gs, int hist_maxd, Assembly &A) {
     s.g = g.groups;
     s.c = g.added_centers;
     s.flags = g.flags;
-    s.bits = choose_bits(s.n, s.d, s.b, s.g, s.c, count);
+    // Persistent batches run every graph on the instance of the largest.
+    const int sized = A.persistent ? n_max : s.n;
+    s.bits = std::min(bc.bits, max_bits_for(sized));
     if (s.bits == 0) return set_error(GG_ERR_TOO_LARGE, "n=%d does not fit LDS", s.n);
-    block_vpt_for(s.n, &s.block, &s.vpt);
+    block_vpt_pref(sized, bc.block, &s.block, &s.vpt);
     if (!kernel_for({s.bits, s.block, s.vpt, 0}))
       return set_error(GG_ERR_TOO_LARGE, "no %d-bit instance for n=%d", s.bits, s.n);
     s.adj_off = A.adj_elems;
@@ -973,6 +1085,7 @@ int assemble(int count, const gg_graph *gs, int hist_maxd, Assembly &A) {
     lg.qcap = qcap_for(kv.first.bits, lg.n_cap, budget);
     if (lg.qcap < 64) return set_error(GG_ERR_TOO_LARGE, "LDS budget too small");
     lg.lds = lds_bytes_for(kv.first.bits, lg.n_cap, lg.qcap);
+    lg.grid = 0;
     lg.unit_begin = (int)A.units.size();
     for (int i : ids) {
       const Slot &s = A.slots[i];
@@ -987,6 +1100,15 @@ int assemble(int count, const gg_graph *gs, int hist_maxd, Assembly &A) {
       m.unit_count = (int)A.units.size() - m.unit_begin;
     }
     lg.unit_count = (int)A.units.size() - lg.unit_begin;
+    if (A.persistent) {
+      const QueueKernel qk = queue_kernel_for(lg.key);
+      if (!qk) return set_error(GG_ERR_STATE, "no persistent instance for %d-bit words", lg.key.bits);
+      int per_cu = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)qk, lg.key.block,
+                                                       lg.lds + 16) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+      lg.grid = std::max(1, std::min(lg.unit_count, per_cu * E.cus));
+    }
     A.groups.push_back(lg);
   }
   return GG_OK;
@@ -1013,7 +1135,7 @@ int prepare_buffers(Assembly &A, DevSet &D, int hist_maxd, int32_t *hist_dev,
   // +64 bytes: the pull levels always load a row's first two 16-byte chunks,
   // so a one-chunk last row reads 16 bytes past the table.
   if ((rc = D.adjT.ensure(A.adjT_elems * 2 + 64))) return rc;
-  if ((rc = D.flags.ensure((size_t)count * 2 * 4))) return rc;
+  if ((rc = D.flags.ensure(((size_t)count * 2 + 1) * 4))) return rc;  // + the queue counter
   if ((rc = D.gdesc.ensure((size_t)count * sizeof(GraphDev)))) return rc;
   if ((rc = D.units.ensure(A.units.size() * sizeof(UnitDesc)))) return rc;
   if ((rc = D.outs.ensure(A.units.size() * sizeof(UnitOut)))) return rc;
@@ -1086,9 +1208,13 @@ int order_shift(int bits) { return bits == 8 ? 0 : bits == 16 ? 1 : 2; }
 
 // `size` is the graph's source-tile count, or its vertex count for a table
 // ordered once (min_size says which threshold applies).
-bool order_applies(int size, int min_size, int dp) {
+bool order_applies_mode(int mode, int size, int min_size, int dp) {
   if (dp > 32 || dp % 8) return false;
-  return g_row_order == 2 || (g_row_order == 1 && size >= min_size);
+  return mode == 2 || (mode == 1 && size >= min_size);
+}
+
+bool order_applies(int size, int min_size, int dp) {
+  return order_applies_mode(g_row_order, size, min_size, dp);
 }
 
 int upload_order(Assembly &A, DevSet &D) {
@@ -1098,8 +1224,8 @@ int upload_order(Assembly &A, DevSet &D) {
   A.order_max_n = 0;
   for (int i = 0; i < count; ++i) {
     const Slot &s = A.slots[i];
-    const int shift =
-        order_applies(A.meta[i].unit_count, kOrderMinTiles, s.dp) ? order_shift(s.bits) : -1;
+    const int shift = order_applies_mode(A.order_mode, A.meta[i].unit_count, kOrderMinTiles, s.dp)
+                          ? order_shift(s.bits) : -1;
     h[i] = {(uint16_t *)D.adjT.p + s.adjT_off, nullptr, s.n, s.dp, shift};
     if (shift >= 0) {
       A.order_nch_mask |= 1 << (s.dp / 8);
@@ -1163,8 +1289,8 @@ int enqueue(Assembly &A, DevSet &D, const int32_t *const *adj_src, const int32_t
   // flags cleared by a kernel-free path: hipMemsetAsync is avoided so the
   // sequence can be captured (a small fill is folded into the prep launch
   // ordering below through a dedicated clear kernel).
-  hipLaunchKernelGGL(gg_clear_i32_kernel, dim3((2 * count + 255) / 256), dim3(256), 0, E.stream,
-                     (int32_t *)D.flags.p, 2 * count);
+  hipLaunchKernelGGL(gg_clear_i32_kernel, dim3((2 * count + 1 + 255) / 256), dim3(256), 0, E.stream,
+                     (int32_t *)D.flags.p, 2 * count + 1);
   if (patched) {
     // All slots share one shape (checked at plan creation) and one base table.
     const Slot &s = A.slots[0];
@@ -1194,6 +1320,15 @@ int enqueue(Assembly &A, DevSet &D, const int32_t *const *adj_src, const int32_t
     launch_order((const OrderDesc *)D.order.p, count, A.order_max_n, A.order_nch_mask, E.stream);
   if (ev_begin) GG_HIP(hipEventRecord(ev_begin, E.stream));
   for (const LaunchGroup &lg : A.groups) {
+    if (A.persistent) {
+      const QueueKernel qk = queue_kernel_for(lg.key);
+      if (!qk) return set_error(GG_ERR_STATE, "no persistent instance bits=%d", lg.key.bits);
+      hipLaunchKernelGGL(qk, dim3(lg.grid), dim3(lg.key.block), lg.lds + 16, E.stream,
+                         (const GraphDev *)D.gdesc.p, (const UnitDesc *)D.units.p, (UnitOut *)D.outs.p,
+                         lg.unit_begin, lg.unit_count, lg.n_cap, lg.qcap,
+                         (int *)D.flags.p + 2 * count);
+      continue;
+    }
     TileKernel k = kernel_for(lg.key);
     if (!k) return set_error(GG_ERR_STATE, "no kernel instance bits=%d block=%d vpt=%d", lg.key.bits,
                              lg.key.block, lg.key.vpt);
@@ -1252,7 +1387,7 @@ static uint64_t *g_trace_host = nullptr;
 // one adjacency upload, one graph launch and one wait.
 struct SingleCache {
   bool valid = false;
-  int n = 0, d = 0, b = 0, g = 0, c = 0, flags = 0, has_rel = 0, bits = 0, order = 0;
+  int n = 0, d = 0, b = 0, g = 0, c = 0, flags = 0, has_rel = 0, bits = 0, order = 0, block = 0;
   Assembly A;
   DevSet D;
   std::vector<const int32_t *> src, rel;
@@ -1278,11 +1413,13 @@ constexpr size_t kStageChunks = 4;
 
 int eval_single_cached(const gg_graph &g, gg_result *out) {
   SingleCache &C = g_cache;
-  const int bits = choose_bits(g.nodes, g.degree, g.based_nodes, g.groups, g.added_centers, 1);
+  const Config cfg = config_for(g.nodes, g.degree, g.based_nodes, g.groups, g.added_centers, false);
+  const int bits = cfg.bits, order = effective_order(cfg);
   const int has_rel = g.relabel != nullptr;
   const bool same = C.valid && C.n == g.nodes && C.d == g.degree && C.b == g.based_nodes &&
                     C.g == g.groups && C.c == g.added_centers && C.flags == g.flags &&
-                    C.has_rel == has_rel && C.bits == bits && C.order == g_row_order;
+                    C.has_rel == has_rel && C.bits == bits && C.order == order &&
+                    C.block == cfg.block;
   int rc;
   if (!same) {
     C.drop();
@@ -1311,7 +1448,8 @@ int eval_single_cached(const gg_graph &g, gg_result *out) {
     C.flags = g.flags;
     C.has_rel = has_rel;
     C.bits = bits;
-    C.order = g_row_order;
+    C.order = order;
+    C.block = cfg.block;
     C.valid = true;
   }
   const Slot &s = C.A.slots[0];
@@ -1433,10 +1571,11 @@ int eval_noalloc(const gg_graph &g, gg_result *out) {
   if (!base) return set_error(GG_ERR_STATE, "scratch array address unknown on device %d", E.device);
   const int n = g.nodes, d = g.degree, dp = (d + 7) & ~7;
   const int B = g.based_nodes, c = g.added_centers;
-  const int bits = choose_bits(n, d, B, g.groups, c, 1);
+  const Config cfg = config_for(n, d, B, g.groups, c, false);
+  const int bits = cfg.bits;
   if (bits == 0) return set_error(GG_ERR_TOO_LARGE, "n=%d does not fit LDS", n);
   int block, vpt;
-  block_vpt_for(n, &block, &vpt);
+  block_vpt_pref(n, cfg.block, &block, &vpt);
   const TileKernel k = kernel_for({bits, block, vpt, 0});
   if (!k) return set_error(GG_ERR_TOO_LARGE, "no %d-bit instance for n=%d", bits, n);
   const int n_cap = (n + 7) & ~7;
@@ -1481,7 +1620,8 @@ int eval_noalloc(const gg_graph &g, gg_result *out) {
   gd.hist = nullptr;
   gd.trace = nullptr;
   const PrepDesc pd = {g.adj, g.relabel, adjR, flags, n, d, dp, (g.flags & GG_GRAPH_SYMMETRIC) ? 0 : 1};
-  const int shift = order_applies(units, kOrderMinTiles, dp) ? order_shift(bits) : -1;
+  const int shift =
+      order_applies_mode(effective_order(cfg), units, kOrderMinTiles, dp) ? order_shift(bits) : -1;
   const OrderDesc od = {adjR, nullptr, n, dp, shift};
   const GraphMeta gm = {0, units, n, g.groups};
   const hipStream_t st = E.stream;
@@ -1549,6 +1689,7 @@ void gg_finalize(void) {
   gg::finalize();
 }
 const char *gg_last_error(void) { return gg::last_error(); }
+int gg_last_error_code(void) { return gg::last_error_code(); }
 int gg_device_count(void) {
   int c = 0;
   if (hipGetDeviceCount(&c) != hipSuccess) return 0;
@@ -1698,74 +1839,167 @@ int gg_set_word_bits(int bits) {
   return GG_OK;
 }
 
-// Times every feasible word width on a seeded graph of the given shape
-// (batch copies in one plan) and records the fastest in the tuning table.
+static gg_plan *plan_create(int count, const gg_graph *shapes, int max_patch);
+
+// Times every candidate configuration of `count` graphs (host adjacency as
+// given) through a plan and keeps the fastest in the tuning table under the
+// shape bucket of the largest graph.  Candidates: every word width that
+// fits, 256- or 1024-thread blocks (up to 2048 vertices), row order off or
+// on, and for batches one queue-driven launch or one launch per kernel
+// instance.  Every candidate must return the same (connected, diameter, S)
+// for every graph, else the tuner fails with GG_ERR_STATE.
+static int tune_run(int count, const gg_graph *graphs, int reps, Config *best_cfg, double *best_ms) {
+  if (count < 1 || !graphs || reps < 1) return set_error(GG_ERR_INVALID, "bad tuning arguments");
+  int rc;
+  if ((rc = init(-1))) return rc;
+  int big = 0;
+  for (int i = 0; i < count; ++i) {
+    if ((rc = validate(graphs[i]))) return rc;
+    if (!graphs[i].adj) return set_error(GG_ERR_INVALID, "graph %d has no adjacency", i);
+    if (graphs[i].nodes > graphs[big].nodes) big = i;
+  }
+  const bool batched = count > 1;
+  const int n_max = graphs[big].nodes;
+  const int maxb = max_bits_for(n_max);
+  if (maxb == 0) return set_error(GG_ERR_TOO_LARGE, "n=%d does not fit LDS", n_max);
+  std::vector<Config> cands;
+  for (int bits : {8, 16, 32, 64}) {
+    if (bits > maxb) continue;
+    for (int block : {0, 1024}) {
+      if (block == 1024 && n_max > 2048) continue;  // 1024 is the only choice there
+      for (int order : {0, 2})
+        for (int pers : {1, 0}) {
+          if (pers && !batched) continue;
+          cands.push_back({bits, block, order, pers});
+        }
+    }
+  }
+  const bool saved_on = E.forced_cfg;
+  const Config saved = E.forced;
+  const int saved_bits = E.forced_bits;
+  E.forced_bits = 0;
+  std::vector<gg_result> ref(count), res(count);
+  bool have_ref = false;
+  double best_t = 1e30;
+  Config best{0, 0, -1, 0};
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  rc = GG_OK;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    rc = set_error(GG_ERR_HIP, "event creation failed");
+  for (size_t k = 0; k < cands.size() && rc == GG_OK; ++k) {
+    E.forced_cfg = true;
+    E.forced = cands[k];
+    gg_plan *p = plan_create(count, graphs, 0);
+    E.forced_cfg = saved_on;
+    E.forced = saved;
+    if (!p) {
+      rc = last_error_code() ? last_error_code() : GG_ERR_STATE;
+      break;
+    }
+    for (int i = 0; i < count && rc == GG_OK; ++i) rc = gg_plan_set_adj(p, i, graphs[i].adj);
+    if (rc == GG_OK) rc = gg_plan_run(p, res.data());  // warm-up
+    if (rc == GG_OK) {
+      hipEventRecord(e0, E.stream);
+      for (int r = 0; r < reps && rc == GG_OK; ++r) rc = gg_plan_launch(p);
+      hipEventRecord(e1, E.stream);
+      if (rc == GG_OK) rc = gg_plan_wait(p, res.data());
+    }
+    float ms = 0;
+    if (rc == GG_OK && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+      rc = set_error(GG_ERR_HIP, "timing failed");
+    gg_plan_destroy(p);
+    if (rc) break;
+    for (int i = 0; i < count; ++i) {
+      if (!have_ref) continue;
+      if (res[i].connected != ref[i].connected || res[i].diameter != ref[i].diameter ||
+          (res[i].connected && res[i].sum != ref[i].sum))
+        rc = set_error(GG_ERR_STATE, "tuning candidate %d-bit/%d/%d/%d disagrees on graph %d", cands[k].bits,
+                       cands[k].block, cands[k].order, cands[k].persistent, i);
+    }
+    if (!have_ref) {
+      ref = res;
+      have_ref = true;
+    }
+    const double t = ms / reps;
+    if (getenv("GG_TUNE_VERBOSE"))
+      fprintf(stderr, "gg tune: %d graphs, bits %d block %d order %d persistent %d: %.4f ms\n", count,
+              cands[k].bits, cands[k].block, cands[k].order, cands[k].persistent, t);
+    if (rc == GG_OK && t < best_t) {
+      best_t = t;
+      best = cands[k];
+    }
+  }
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  E.forced_bits = saved_bits;
+  if (rc) return rc;
+  const gg_graph &g = graphs[big];
+  E.tune[{bucket_of(g.nodes), g.degree, g.groups, g.added_centers > 0 ? 1 : 0, batched ? 1 : 0}] = best;
+  if (best_cfg) *best_cfg = best;
+  if (best_ms) *best_ms = best_t;
+  return GG_OK;
+}
+
+// A seeded graph of the given shape (`batch` copies of it) through tune_run.
 int gg_autotune(int nodes, int degree, int based_nodes, int groups, int batch, int reps,
                 int *best_bits, double *best_ms) {
   if (nodes < 2 || degree < 1 || batch < 1 || reps < 1 || groups < 1 || based_nodes < 1)
     return set_error(GG_ERR_INVALID, "bad autotune arguments");
-  int rc = gg::init(-1);
-  if (rc) return rc;
   const int c = nodes - based_nodes * groups;
   std::vector<int32_t> edges((size_t)nodes * degree);
-  rc = groups > 1 && c == 0 ? gg_random_symmetric(nodes, degree, groups, 12345, edges.data())
-                            : gg_random_regular(nodes, degree, 12345, edges.data());
+  int rc = groups > 1 && c == 0 ? gg_random_symmetric(nodes, degree, groups, 12345, edges.data())
+                                : gg_random_regular(nodes, degree, 12345, edges.data());
   if (rc) return rc;
   std::vector<int32_t> adj((size_t)nodes * degree);
   if ((rc = gg_edges_to_adj(nodes, nodes * degree / 2, degree, edges.data(), adj.data()))) return rc;
   gg_graph g = {nodes, degree, based_nodes, groups, c, GG_GRAPH_SYMMETRIC, adj.data(), nullptr};
   std::vector<gg_graph> shapes(batch, g);
-  std::vector<gg_result> res(batch);
-  const int saved = engine().forced_bits;
-  int best = 0;
-  double best_t = 1e30;
-  for (int bits : {8, 16, 32, 64}) {
-    if (bits > max_bits_for(nodes)) continue;
-    engine().forced_bits = bits;
-    gg_plan *p = gg_plan_create(batch, shapes.data());
-    if (!p) {
-      engine().forced_bits = saved;
-      return GG_ERR_HIP;
-    }
-    for (int i = 0; i < batch; ++i) gg_plan_set_adj(p, i, adj.data());
-    rc = gg_plan_run(p, res.data());  // warm-up
-    hipEvent_t e0, e1;
-    hipEventCreate(&e0);
-    hipEventCreate(&e1);
-    hipEventRecord(e0, engine().stream);
-    for (int r = 0; r < reps && !rc; ++r) rc = gg_plan_launch(p);
-    hipEventRecord(e1, engine().stream);
-    if (!rc) rc = gg_plan_wait(p, res.data());
-    float ms = 0;
-    hipEventSynchronize(e1);
-    hipEventElapsedTime(&ms, e0, e1);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
-    gg_plan_destroy(p);
-    if (rc) {
-      engine().forced_bits = saved;
-      return rc;
-    }
-    double t = ms / reps;
-    if (t < best_t) {
-      best_t = t;
-      best = bits;
-    }
+  Config best;
+  if ((rc = tune_run(batch, shapes.data(), reps, &best, best_ms))) return rc;
+  if (best_bits) *best_bits = best.bits;
+  return GG_OK;
+}
+
+int gg_autotune_batch(int count, const gg_graph *graphs, int reps, int32_t *config, double *best_ms) {
+  Config best;
+  const int rc = tune_run(count, graphs, reps, &best, best_ms);
+  if (rc) return rc;
+  if (config) {
+    config[0] = best.bits;
+    config[1] = best.block;
+    config[2] = best.order;
+    config[3] = best.persistent;
   }
-  engine().forced_bits = saved;
-  engine().tune[{nodes, degree, based_nodes, groups, batch > 1 ? 1 : 0}] = best;
-  if (best_bits) *best_bits = best;
-  if (best_ms) *best_ms = best_t;
+  return GG_OK;
+}
+
+int gg_tune_lookup(int nodes, int degree, int based_nodes, int groups, int added_centers, int batched,
+                   int32_t *config) {
+  if (!config || nodes < 2 || degree < 1 || groups < 1 || based_nodes < 1 || added_centers < 0)
+    return set_error(GG_ERR_INVALID, "bad lookup arguments");
+  const Config c = config_for(nodes, degree, based_nodes, groups, added_centers, batched != 0);
+  if (c.bits == 0) return set_error(GG_ERR_TOO_LARGE, "n=%d does not fit LDS", nodes);
+  int block, vpt;
+  block_vpt_pref(nodes, c.block, &block, &vpt);
+  config[0] = c.bits;
+  config[1] = block;
+  config[2] = effective_order(c);
+  config[3] = c.persistent;
+  return GG_OK;
+}
+
+int gg_tune_table_clear(void) {
+  engine().tune.clear();
   return GG_OK;
 }
 
 int gg_tune_table_save(const char *path) {
   FILE *f = fopen(path, "w");
   if (!f) return set_error(GG_ERR_INVALID, "cannot write %s", path);
-  fprintf(f, "# nodes degree based_nodes groups batched word_bits\n");
+  fprintf(f, "# n_bucket degree groups centres batched word_bits block row_order persistent\n");
   for (auto &kv : engine().tune)
-    fprintf(f, "%d %d %d %d %d %d\n", kv.first.n, kv.first.d, kv.first.b, kv.first.g,
-            kv.first.batch, kv.second);
+    fprintf(f, "%d %d %d %d %d %d %d %d %d\n", kv.first.nb, kv.first.d, kv.first.g, kv.first.c,
+            kv.first.batched, kv.second.bits, kv.second.block, kv.second.order, kv.second.persistent);
   fclose(f);
   return GG_OK;
 }
@@ -1777,10 +2011,11 @@ int gg_tune_table_load(const char *path) {
   int loaded = 0;
   while (fgets(line, sizeof line, f)) {
     if (line[0] == '#') continue;
-    int n, d, b, g, bt, bits;
-    if (sscanf(line, "%d %d %d %d %d %d", &n, &d, &b, &g, &bt, &bits) == 6 &&
-        (bits == 8 || bits == 16 || bits == 32 || bits == 64)) {
-      engine().tune[{n, d, b, g, bt}] = bits;
+    int nb, d, g, c, bt, bits, block, order, pers;
+    if (sscanf(line, "%d %d %d %d %d %d %d %d %d", &nb, &d, &g, &c, &bt, &bits, &block, &order, &pers) == 9 &&
+        (bits == 8 || bits == 16 || bits == 32 || bits == 64) && (block == 0 || block == 256 || block == 1024) &&
+        order >= -1 && order <= 2 && (pers == 0 || pers == 1)) {
+      engine().tune[{nb, d, g, c ? 1 : 0, bt ? 1 : 0}] = Config{bits, block, order, pers};
       ++loaded;
     }
   }
@@ -1959,6 +2194,16 @@ float gg_plan_kernel_ms(gg_plan *p) {
   hipEvent_t b = p->timed ? p->tev_begin : p->ev_begin, e = p->timed ? p->tev_end : p->ev_end;
   if (!b || hipEventElapsedTime(&ms, b, e) != hipSuccess) return -1.0f;
   return ms;
+}
+
+int gg_plan_config(gg_plan *p, int32_t *config) {
+  if (!p || !config) return set_error(GG_ERR_INVALID, "bad plan config arguments");
+  const Slot &s = p->A.slots[0];
+  config[0] = s.bits;
+  config[1] = s.block;
+  config[2] = p->A.order_mode;
+  config[3] = p->A.persistent ? 1 : 0;
+  return GG_OK;
 }
 
 int gg_synchronize(void) {

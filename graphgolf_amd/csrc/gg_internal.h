@@ -4,5 +4,6 @@
 namespace gg {
 int set_error(int code, const char *fmt, ...);
 const char *last_error();
+int last_error_code();
 }  // namespace gg
 #endif

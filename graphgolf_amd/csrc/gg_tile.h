@@ -151,7 +151,14 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
 
 // ---------------------------------------------------------------------------
 // The tile BFS.  One workgroup = one UnitDesc.  LDS layout (dynamic):
-//   R[n_cap] (W) | qb[2][qcap] (W) | qv[2][qcap] (u16) | red[2][16][4] | qcount[2]
+//   R[n_cap] (W) | qb[2][qcap] (W) | qv[2][qcap] (u16) | red[2][16][4] |
+//   qcount[4] | sacc[2][64]
+// tile_lds_bytes gives its size for `wb`-byte words.
+__host__ __device__ constexpr size_t tile_align16(size_t x) { return (x + 15) & ~(size_t)15; }
+__host__ __device__ constexpr size_t tile_lds_bytes(size_t wb, int n_cap, int qcap) {
+  return tile_align16((size_t)n_cap * wb) + tile_align16((size_t)2 * qcap * wb) +
+         tile_align16((size_t)2 * qcap * 2) + 2 * 16 * 4 * 4 + 16 + 2 * 64 * 4;
+}
 
 // 16 bytes of an adjacency row: 8 uint16 neighbour ids.
 typedef uint32_t Chunk __attribute__((ext_vector_type(4)));
@@ -205,6 +212,37 @@ gg_tile_bfs_kernel(const GraphDev *__restrict__ graphs, const UnitDesc *__restri
 #define GG_UNIT_EXIT return
 #include "gg_tile_body.inc"
 #undef GG_UNIT_EXIT
+}
+
+// Persistent variant for batches: every workgroup takes (graph, tile) units
+// from a device-side counter until `unit_count` are taken, so one launch
+// drains the whole batch (units are listed largest graph first, so the
+// last units to start are the cheapest).  *counter must be 0 at launch.
+// The unit index is handed to the workgroup through the 16 bytes after the
+// body's LDS layout (a static __shared__ variable would move the dynamic
+// LDS, whose reach array must sit at address 0); the launch adds them.
+template <typename W, int BLOCK, int VPT, int MODE>
+__global__ void __launch_bounds__(BLOCK)
+gg_tile_queue_kernel(const GraphDev *__restrict__ graphs, const UnitDesc *__restrict__ units,
+                     UnitOut *__restrict__ outs, int unit_base, int unit_count, int n_cap, int qcap,
+                     int *__restrict__ counter) {
+  extern __shared__ __align__(16) unsigned char smem_q[];
+  volatile int *s_next = (volatile int *)(smem_q + tile_lds_bytes(sizeof(W), n_cap, qcap));
+  for (;;) {
+    __syncthreads();  // LDS (and the slot) is reused from the previous unit
+    if (threadIdx.x == 0) *s_next = atomicAdd(counter, 1);
+    __syncthreads();
+    // Uniform (one value for the workgroup): readfirstlane keeps the unit's
+    // descriptors in scalar registers, as in the one-unit kernels.
+    const int k = __builtin_amdgcn_readfirstlane(*s_next);
+    if (k >= unit_count) break;
+    const int uidx = unit_base + k;
+    const UnitDesc U = units[uidx];
+    const GraphDev G = graphs[U.graph];
+#define GG_UNIT_EXIT continue
+#include "gg_tile_body.inc"
+#undef GG_UNIT_EXIT
+  }
 }
 
 // Persistent variant for the source-list modes over graphs[0]: units are

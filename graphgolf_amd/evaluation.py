@@ -124,7 +124,8 @@ class Plan:
             self.nodes.append(nodes)
         p = L.lib().gg_plan_create(self.n, arr)
         if not p:
-            raise L.GGError(-4, (L.lib().gg_last_error() or b"").decode())
+            raise L.GGError(L.lib().gg_last_error_code() or -4,
+                            (L.lib().gg_last_error() or b"").decode())
         self._p = ctypes.c_void_p(p)
         self._out = (L.GGResult * self.n)()
 
@@ -157,6 +158,12 @@ class Plan:
         """Tile-BFS kernel time of the last completed replay (HIP events)."""
         return float(L.lib().gg_plan_kernel_ms(self._p))
 
+    def config(self) -> dict:
+        """The launch configuration the plan was built with."""
+        cfg = np.zeros(4, dtype=np.int32)
+        L.check(L.lib().gg_plan_config(self._p, cfg.ctypes.data_as(L.i32p)))
+        return dict(zip(CONFIG_KEYS, (int(x) for x in cfg)))
+
     def close(self) -> None:
         if getattr(self, "_p", None):
             L.lib().gg_plan_destroy(self._p)
@@ -186,10 +193,15 @@ def synchronize() -> None:
     L.check(L.lib().gg_synchronize())
 
 
+CONFIG_KEYS = ("word_bits", "block", "row_order", "persistent")
+
+
 def autotune(nodes: int, degree: int, based_nodes=None, groups: int = 1, batch: int = 1,
              reps: int = 5):
-    """Time every feasible word width for this shape; the fastest is used
-    from then on.  Returns (bits, ms per replay)."""
+    """Time every candidate configuration (word width, block, row order,
+    persistent batch launch) on a seeded graph of this shape (`batch` copies);
+    the fastest is used for the shape's bucket from then on.  Returns
+    (bits, ms per replay)."""
     bits = ctypes.c_int(0)
     ms = ctypes.c_double(0)
     b = nodes // groups if based_nodes is None else based_nodes
@@ -198,8 +210,39 @@ def autotune(nodes: int, degree: int, based_nodes=None, groups: int = 1, batch: 
     return bits.value, ms.value
 
 
+def autotune_batch(graphs: Sequence[dict], reps: int = 3):
+    """Tune on the caller's graphs (items as eval_batch); the winner is kept
+    under the bucket of the largest.  Returns (config dict, ms per replay)."""
+    keep = []
+    arr = (L.GGGraph * len(graphs))()
+    for i, item in enumerate(graphs):
+        adj = L.c32(item["adj"])
+        keep.append(adj)
+        arr[i] = _graph(adj, item.get("based_nodes"), item.get("groups", 1),
+                        item.get("added_centers", 0), None, item.get("symmetric", False))
+    cfg = np.zeros(4, dtype=np.int32)
+    ms = ctypes.c_double(0)
+    L.check(L.lib().gg_autotune_batch(len(graphs), arr, reps, cfg.ctypes.data_as(L.i32p),
+                                      ctypes.byref(ms)))
+    return dict(zip(CONFIG_KEYS, (int(x) for x in cfg))), ms.value
+
+
+def tune_lookup(nodes: int, degree: int, based_nodes=None, groups: int = 1, added_centers: int = 0,
+                batched: bool = False) -> dict:
+    """The configuration an evaluation (or a batch led by this shape) uses now."""
+    cfg = np.zeros(4, dtype=np.int32)
+    b = (nodes - added_centers) // groups if based_nodes is None else based_nodes
+    L.check(L.lib().gg_tune_lookup(nodes, degree, b, groups, added_centers, int(batched),
+                                   cfg.ctypes.data_as(L.i32p)))
+    return dict(zip(CONFIG_KEYS, (int(x) for x in cfg)))
+
+
 def tune_table_save(path: str) -> None:
     L.check(L.lib().gg_tune_table_save(path.encode()))
+
+
+def tune_table_clear() -> None:
+    L.check(L.lib().gg_tune_table_clear())
 
 
 def tune_table_load(path: str) -> int:

@@ -76,6 +76,7 @@ typedef struct gg_result {
 int gg_init(int device);
 void gg_finalize(void);
 const char *gg_last_error(void);
+int gg_last_error_code(void); /* the GG_ERR_* code that came with gg_last_error() */
 int gg_device_count(void);
 
 /* One evaluation, adjacency in host memory (copied to the device each call,
@@ -143,14 +144,32 @@ float gg_plan_kernel_ms(gg_plan *p);
 /* Waits for all work of the evaluator's stream and the device. */
 int gg_synchronize(void);
 
-/* ---- Tuning.  Word width (8/16/32/64 sources per LDS word) and LDS budget
- * per graph shape.  gg_set_word_bits(0) restores automatic choice (autotuned
- * table when loaded, else the built-in heuristic). */
+/* ---- Tuning.  A launch configuration is (word bits 8/16/32/64, threads per
+ * block 0 = from n / 256 / 1024, row-order mode 0 off / 1 by size / 2 on,
+ * persistent: a batch in one queue-driven launch).  Configurations are kept
+ * per shape bucket (ceil(log2 n) of the graph, or of a batch's largest graph;
+ * degree; groups; centres or not; batch or single) and consulted by every
+ * evaluation and plan made afterwards; without an entry the built-in choice
+ * applies.  gg_set_word_bits(b) forces the word width (0 restores the
+ * automatic choice); gg_set_row_order (0 or 2) wins over the table's order.
+ * config[4] = {bits, block, row order, persistent}. */
 int gg_set_word_bits(int bits);
+/* Times every candidate on a seeded graph of the shape (`batch` copies in one
+ * plan) and records the fastest; all candidates must agree bit for bit. */
 int gg_autotune(int nodes, int degree, int based_nodes, int groups, int batch, int reps,
                 int *best_bits, double *best_ms);
+/* The same on the caller's `count` graphs (host adjacency), recorded under the
+ * bucket of the largest; config may be NULL. */
+int gg_autotune_batch(int count, const gg_graph *graphs, int reps, int32_t *config, double *best_ms);
+/* The configuration an evaluation (batched = 0) or a batch whose largest graph
+ * has this shape (batched = 1) would use now. */
+int gg_tune_lookup(int nodes, int degree, int based_nodes, int groups, int added_centers, int batched,
+                   int32_t *config);
+/* The configuration a plan was built with (slot 0's word bits and block). */
+int gg_plan_config(gg_plan *p, int32_t *config);
 int gg_tune_table_load(const char *path); /* returns entries loaded, or an error code */
 int gg_tune_table_save(const char *path);
+int gg_tune_table_clear(void);
 
 /* ---- Per-vertex distance features (eccentricity / distance histogram) from
  * row popcounts of the same bit-parallel BFS.  hist is [nodes][maxd+1] int32:
