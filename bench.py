@@ -183,30 +183,46 @@ def cpu_baseline(items, budget_s):
 
 
 def sa_time_to_aspl(budget_s):
-    """SA time-to-ASPL at n=65536 d=16 (random regular, seed 5): plain
-    single-graph GPU SA runs for budget_s; surrogate-ranked SA then runs until
-    its best ASPL reaches plain's final best.  Every accepted graph in both is
-    scored exactly by the evaluator."""
+    """SA time-to-ASPL at n=65536 d=16 (random regular, seed 5).  Plain
+    single-graph GPU SA runs N steps, N sized so that it takes about budget_s
+    (a short calibration run measures its time per step), with the
+    reference's cooling from 100 to 0.217147 over those N steps
+    (general/main.c:441, general/sa.c:509); its final best ASPL is the
+    target.  Then, on the same schedule and seed: plain SA scored by the
+    incremental scorer, surrogate-ranked SA, and the same batched SA with
+    random instead of learned ranking (what merging vertex-disjoint moves
+    gains without the network), each until it reaches the target.  Every
+    accepted graph in all of them is scored exactly by the evaluator."""
     from graphgolf_amd import graphs as G
     from graphgolf_amd import sa as SA
 
     n, d = 65536, 16
     e0 = G.random_regular(n, d, 5)
-    plain = SA.sa_batched(e0, n, d, max_steps=10**9, time_budget_s=budget_s,
-                          use_surrogate=False, exact=1, seed=1)
+    cal = SA.sa_batched(e0, n, d, max_steps=300, use_surrogate=False, exact=1, seed=1)
+    per_step = float(cal.stats["seconds_total"]) / max(1, int(cal.stats["steps"]))
+    steps = max(1000, int(budget_s / per_step))
+    common = dict(max_steps=steps, max_temp=100.0, min_temp=0.217147, seed=1)
+    plain = SA.sa_batched(e0, n, d, use_surrogate=False, exact=1, time_budget_s=3 * budget_s,
+                          **common)
     target = float(plain.log_best_aspl[-1])
     t_plain = float(plain.log_time[int(np.argmax(plain.log_best_aspl <= target))])
+    cap = 2 * budget_s
     # The same plain run (same seed, hence the same moves and decisions) with
     # every proposal scored by the incremental scorer.
-    inc = SA.sa_batched(e0, n, d, max_steps=10**9, time_budget_s=budget_s, target_aspl=target,
-                        use_surrogate=False, exact=1, seed=1, incremental=True)
+    inc = SA.sa_batched(e0, n, d, use_surrogate=False, exact=1, incremental=True,
+                        target_aspl=target, time_budget_s=cap, **common)
     t_inc = float(inc.stats["seconds_to_target"])
-    sur = SA.sa_batched(e0, n, d, max_steps=10**9, time_budget_s=budget_s, target_aspl=target,
-                        proposals=12288, combine=128, warmup=100, explore_every=16, seed=1,
-                        incremental=True)
+    ranked = dict(proposals=12288, combine=128, warmup=100, explore_every=16, incremental=True,
+                  target_aspl=target, time_budget_s=cap)
+    sur = SA.sa_batched(e0, n, d, **ranked, **common)
     t_sur = float(sur.stats["seconds_to_target"])
+    rnd = SA.sa_batched(e0, n, d, random_ranking=True, **ranked, **common)
+    t_rnd = float(rnd.stats["seconds_to_target"])
     return {"workload": "sa n=65536 d=16 (random regular start, seed 5)",
+            "schedule": f"T 100 -> 0.217147 geometric over {steps} steps "
+                        f"({per_step * 1e3:.3f} ms per plain step measured)",
             "target_aspl": target, "plain_seconds": round(t_plain, 3),
+            "plain_total_seconds": round(float(plain.stats["seconds_total"]), 3),
             "plain_evaluations": int(plain.stats["evaluations"]),
             "plain_incremental_seconds": round(t_inc, 3) if t_inc > 0 else None,
             "plain_incremental_same_run": bool(inc.stats["evaluations"] <= plain.stats["evaluations"]
@@ -215,7 +231,11 @@ def sa_time_to_aspl(budget_s):
             "surrogate_evaluations": int(sur.stats["evaluations"]),
             "surrogate_moves_accepted": int(sur.stats["moves_accepted"]),
             "surrogate_best_aspl": float(sur.stats["best_aspl"]),
-            "speedup": round(t_plain / t_sur, 2) if t_sur > 0 else None}
+            "random_ranking_seconds": round(t_rnd, 3) if t_rnd > 0 else None,
+            "random_ranking_best_aspl": float(rnd.stats["best_aspl"]),
+            "random_ranking_evaluations": int(rnd.stats["evaluations"]),
+            "speedup": round(t_plain / t_sur, 2) if t_sur > 0 else None,
+            "speedup_random_ranking": round(t_plain / t_rnd, 2) if t_rnd > 0 else None}
 
 
 def incremental_bench(args):
@@ -333,6 +353,9 @@ def main():
     ap.add_argument("--word-bits", type=int, default=0)
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (the built-in configuration runs)")
+    ap.add_argument("--save-tune", default="", help="save the tuned table to this file")
+    ap.add_argument("--tune-table", default="",
+                    help="load this tuning table instead of tuning (profiled runs)")
     ap.add_argument("--sa-budget", type=float, default=20.0,
                     help="seconds of plain GPU SA for the time-to-ASPL comparison (c5 only; 0 = skip)")
     args = ap.parse_args()
@@ -352,11 +375,16 @@ def main():
     tuned = None
     if args.word_bits:
         E.set_word_bits(args.word_bits)
+    elif args.tune_table:
+        E.tune_table_load(args.tune_table)  # a table saved by an earlier run
+        tuned = "table"
     elif not args.no_tune:
         # The autotuner times its candidate configurations on the workload's
         # own graphs and keeps the fastest for their shape bucket; the plan
         # below runs that choice.
         tuned, _ = E.autotune_batch(items, reps=3)
+        if args.save_tune:
+            E.tune_table_save(args.save_tune)
     plan = E.Plan([dict(nodes=it["adj"].shape[0], degree=it["adj"].shape[1],
                         based_nodes=it["based_nodes"], groups=it["groups"],
                         relabel=it.get("relabel"), symmetric=it["symmetric"]) for it in items])

@@ -165,6 +165,31 @@ def test_gpu_batched_grid_sa_respects_length_and_is_exact(surrogate):
         assert np.all(np.abs(x[:, 0] - x[:, 1]) + np.abs(y[:, 0] - y[:, 1]) <= r)
 
 
+def test_gpu_grid_batch_of_256_proposals():
+    """The grid workload at full size: 256 length-respecting 2-opt
+    proposals of a seeded 100x100 d=4 r=2 grid scored in one plan; all 256
+    equal single evaluations, 8 of them the CPU oracle."""
+    from graphgolf_amd import workloads as WL
+
+    print("grid batch 100x100 x256", flush=True)
+    _, variants = WL.grid_batch(100, 100, 4, 2, 1, 256, 4)
+    adjs = [G.edges_to_adj(10000, 4, e) for e in variants]
+    for e in variants:
+        x, y = e // 100, e % 100
+        assert np.all(np.abs(x[:, 0] - x[:, 1]) + np.abs(y[:, 0] - y[:, 1]) <= 2)
+    plan = E.Plan([dict(nodes=10000, degree=4, symmetric=True) for _ in adjs])
+    for i, a in enumerate(adjs):
+        plan.set_adj(i, a)
+    res = plan.run()
+    plan.close()
+    for a, r in zip(adjs, res):
+        s = E.apsp(a)
+        assert (r.connected, r.diameter, r.sum) == (s.connected, s.diameter, s.sum)
+    for k in range(0, 256, 32):
+        o = oracle.matrix_op(adjs[k])
+        assert res[k].connected == o.ok and (res[k].diameter, res[k].sum) == (o.diameter, o.S)
+
+
 @pytest.mark.parametrize("n,d,exact,grid", [(4096, 16, 1, 0), (2048, 8, 4, 0), (900, 4, 1, 30)])
 def test_gpu_plain_sa_incremental_is_the_same_run(n, d, exact, grid):
     """Plain batched SA scored by the incremental scorer makes the same run:

@@ -45,7 +45,7 @@ def apply_move(adj, rem, add):
     return out
 
 
-def check_run(n, d, seed, moves, p_accept, use_oracle=False):
+def check_run(n, d, seed, moves, p_accept, use_oracle=False, oracle_every=0):
     rng = np.random.default_rng(seed)
     adj = G.edges_to_adj(n, d, G.random_regular(n, d, seed))
     full = E.apsp(adj)
@@ -64,6 +64,9 @@ def check_run(n, d, seed, moves, p_accept, use_oracle=False):
                 if use_oracle:
                     o = oracle.apsp_plain(moved)
                     assert (got.diameter, got.sum) == (o.diameter, o.S)
+                elif oracle_every and len(affected) % oracle_every == 0:
+                    o = oracle.matrix_op(moved)
+                    assert (got.diameter, got.sum) == (o.diameter, o.S), (n, d, rem, add)
             affected.append(aff)
             if want.connected and rng.random() < p_accept:
                 sc.accept()
@@ -85,8 +88,10 @@ def test_incremental_matches_full_small(n, d, seed):
 
 @pytest.mark.parametrize("n,d,seed", [(4096, 16, 6), (16384, 8, 7)])
 def test_incremental_matches_full_medium(n, d, seed):
+    """Every move against the device's full evaluation, every 5th against
+    the CPU oracle too."""
     print(f"incr n={n} d={d} seed={seed}", flush=True)
-    check_run(n, d, seed, moves=30, p_accept=0.4)
+    check_run(n, d, seed, moves=30, p_accept=0.4, oracle_every=5)
 
 
 def test_incremental_matches_full_c5():
