@@ -315,20 +315,33 @@ def read_traffic(workload):
     return None if v is None else int(v)
 
 
-def lds_roofline(workload, kernel_ms, traffic, canonical_bytes):
+def gather_ceiling_GBps(word_bits, ordered):
+    """The LDS's measured random-gather throughput for these words
+    (profiles/lds_gather_floor.md): cycles per 64-lane read with the 2-way
+    conflicts the bank-aware row order leaves, or for random ids; capped at
+    the LDS width."""
+    if word_bits == 64:
+        cycles = 3.5 if ordered else 6.7
+    else:
+        cycles = 4.237 if ordered else 6.627
+    per_cu = min(128.0, 64 * (word_bits // 8) / cycles)  # B/clk
+    return 256 * 2.4 * per_cu
+
+
+def lds_roofline(workload, kernel_ms, traffic, canonical_bytes, word_bits=16, ordered=True):
     """The tile BFS keeps every reach word in LDS, so LDS is its binding
     memory: LDS bytes per launch (rocprofv3 SQ_INSTS_LDS_*_BANDWIDTH x 64 B,
     from the committed summary of this workload) over this run's kernel time
     (HIP events on the evaluator's stream), against 256 CUs x 128 B/clk x
-    2.4 GHz.  The random 16-bit gathers cannot use the LDS's full width: the
-    measured gather ceiling (one 64-lane ds_read_u16 per 4.24 cycles per CU
-    with the 2-way bank conflicts the row order leaves) is reported beside
-    it.  `traffic` is the measured HBM bytes per launch; the canonical bytes
-    of the reference's single-tile formulation are kept as
-    `canonical_bytes`."""
+    2.4 GHz.  Random gathers cannot use the LDS's full width: the measured
+    gather ceiling for these words (16-bit: one 64-lane ds_read_u16 per 4.24
+    cycles per CU with the 2-way bank conflicts the row order leaves; 64-bit
+    random ids: one ds_read_b64 per 6.7 cycles) is reported beside it.
+    `traffic` is the measured HBM bytes per launch; the canonical bytes of
+    the reference's single-tile formulation are kept as `canonical_bytes`."""
     name = f"lds_{workload}.md"
     lds_bytes = read_profile_value(name, "lds_bytes_per_launch")
-    gather_ceiling = 256 * 2.4 * (2 * 64 / 4.237)  # GB/s: 128 B per 4.24 cycles per CU
+    gather_ceiling = gather_ceiling_GBps(word_bits, ordered)
     out = {"bound": "lds", "achieved": None, "peak": LDS_PEAK_GBPS, "unit": "GB/s", "frac": None,
            "traffic": traffic, "kernel_ms": round(kernel_ms, 4),
            "lds_bytes_per_launch": None if lds_bytes is None else int(lds_bytes),
@@ -340,6 +353,17 @@ def lds_roofline(workload, kernel_ms, traffic, canonical_bytes):
         out.update(achieved=round(a, 1), frac=round(a / LDS_PEAK_GBPS, 4),
                    gather_ceiling_frac=round(a / gather_ceiling, 4))
     return out
+
+
+def _rows_ordered(cfg, items):
+    """Whether the plan ran the bank-aware row order (mode 2, or mode 1 for
+    graphs of at least 1024 source tiles at this word width)."""
+    if cfg["row_order"] == 2:
+        return True
+    if cfg["row_order"] == 0:
+        return False
+    tiles = min(-(-it["based_nodes"] // cfg["word_bits"]) for it in items)
+    return tiles >= 1024
 
 
 def main():
@@ -444,7 +468,8 @@ def main():
                    "diameter": r0.diameter, "sum": r0.sum, "aspl": r0.aspl,
                    "launch": plan.config(), "tuned": tuned is not None},
         "canonical_effective_GBps": round(achieved, 1),
-        "roofline": lds_roofline(args.workload, k_ms, traffic, total_bytes),
+        "roofline": lds_roofline(args.workload, k_ms, traffic, total_bytes, r0.word_bits,
+                                 _rows_ordered(plan.config(), items)),
         "cpu_baseline": cpu,
         "host_adjacency_ms_per_eval": handover_ms,
     }
