@@ -16,6 +16,7 @@
 #include <cstdint>
 #include <cstring>
 #include <numeric>
+#include <unordered_map>
 #include <vector>
 
 #include "gg_device.h"
@@ -31,7 +32,9 @@ using gg::set_error;
 namespace {
 constexpr int kFeat = 64;        // MLP input width
 constexpr int kVStat = 10;       // per-vertex statistics (see VertexStats)
-constexpr int kMaxCombine = 128;  // moves per candidate (4 patch entries each)
+constexpr int kMaxCombine = 128;  // moves per candidate (4 patch entries per copy)
+constexpr int kMaxGroups = 8;     // rotational copies of a move (groups)
+constexpr int kMaxSlots = 4 * kMaxGroups;
 constexpr int kBlock = 256;
 }  // namespace
 
@@ -289,12 +292,94 @@ gg_bsa_set_lines(const int32_t *__restrict__ upd, int count, int32_t *__restrict
   }
 }
 
+// Symmetric graphs (groups g > 1): a move is the 2g-edge 2-opt of
+// exchange_edge_2opt (general/sa.c:270-300, grid/sa.c:183-206) -- lines t0,
+// t1 and their g-1 rotated copies, every copy re-paired with the same
+// variant.  rotk[i*n + v] is vertex v rotated i times (rotk[0] = identity),
+// rotline[i*lines + t] the line holding line t's edge rotated i times.
+// Drawn like gg_bsa_propose (grids: t1 near t0 as gg_bsa_propose_grid) and
+// redrawn while the lines share a vertex, lie in one orbit, or one of them
+// maps to itself under a rotation (the reference re-pairs such lines with
+// edge_1g_opt; this draw leaves them to other moves).  Valid when the 2g
+// added edges are no loops, pairwise distinct, absent from the current
+// adjacency and, on a grid, within the length.
+__global__ void __launch_bounds__(256)
+gg_bsa_propose_sym(int count, uint64_t seed, uint64_t step, int lines, const int32_t *__restrict__ edges,
+                   const int32_t *__restrict__ adj, int d, int n, int groups,
+                   const int32_t *__restrict__ rotk, const int32_t *__restrict__ rotline,
+                   const int32_t *__restrict__ lineat, int height, int length,
+                   const int32_t *__restrict__ offsets, int noff, int32_t *__restrict__ out,
+                   int32_t *__restrict__ lines_out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const uint64_t base = bsa_mix(seed ^ bsa_mix(step * 0x100000001B3ull + (uint64_t)i));
+  auto near = [&](int u, int v) {
+    if (height <= 0) return true;
+    const int dx = u / height - v / height, dy = u % height - v % height;
+    return (dx < 0 ? -dx : dx) + (dy < 0 ? -dy : dy) <= length;
+  };
+  int t0 = 0, t1 = 0, r = 0, a = 0, b = 0, c = 0, e = 0;
+  bool ok = false;
+  for (int k = 0; k < 64 && !ok; ++k) {
+    const uint64_t x = bsa_mix(base + (uint64_t)k), y = bsa_mix(x);
+    t0 = (int)(((x & 0xffffffffull) * (uint64_t)lines) >> 32);
+    a = edges[2 * t0];
+    b = edges[2 * t0 + 1];
+    if (height > 0) {
+      const int o = offsets[(int)(((x >> 32) * (uint64_t)noff) >> 32)];
+      const int cx = a / height + (o / 4096 - 1024), cy = a % height + (o % 4096 - 1024);
+      if (cx < 0 || cy < 0 || cy >= height || cx >= n / height) continue;
+      t1 = lineat[(size_t)(cx * height + cy) * d + (int)(((y & 0xffffffffull) * (uint64_t)d) >> 32)];
+    } else {
+      t1 = (int)(((x >> 32) * (uint64_t)lines) >> 32);
+    }
+    r = (int)((y >> 63) & 1u);
+    c = edges[2 * t1];
+    e = edges[2 * t1 + 1];
+    ok = t0 != t1 && a != c && a != e && b != c && b != e;
+    for (int q = 1; q < groups && ok; ++q) {
+      const int r0 = rotline[(size_t)q * lines + t0], r1 = rotline[(size_t)q * lines + t1];
+      ok = r0 != t0 && r1 != t1 && r0 != t1;
+    }
+  }
+  bool valid = ok;
+  if (ok) {
+    const int q1 = r ? c : e, p2 = r ? b : c, q2 = r ? e : b;
+    valid = near(a, q1) && near(p2, q2);
+    int nu[2 * kMaxGroups], nv[2 * kMaxGroups];
+    for (int q = 0; q < groups && valid; ++q) {
+      const int32_t *rk = rotk + (size_t)q * n;
+      nu[2 * q] = rk[a];
+      nv[2 * q] = rk[q1];
+      nu[2 * q + 1] = rk[p2];
+      nv[2 * q + 1] = rk[q2];
+      for (int h = 2 * q; h < 2 * q + 2 && valid; ++h)
+        valid = nu[h] != nv[h] && !row_has(adj, d, nu[h], nv[h]);
+    }
+    for (int h = 0; h < 2 * groups && valid; ++h)
+      for (int m = h + 1; m < 2 * groups && valid; ++m)
+        valid = !((nu[h] == nu[m] && nv[h] == nv[m]) || (nu[h] == nv[m] && nv[h] == nu[m]));
+  }
+  int32_t *o = out + 5 * (size_t)i;
+  o[0] = a;
+  o[1] = b;
+  o[2] = c;
+  o[3] = e;
+  o[4] = r;
+  int32_t *l = lines_out + 3 * (size_t)i;
+  l[0] = t0;
+  l[1] = t1;
+  l[2] = valid ? 1 : 0;
+}
+
 struct Prop {
   int t0, t1, r, a, b, c, d;
 };
 
+// Adjacency entries a move rewrites: 4 per rotated copy.
 struct Slots {
-  int at[4], val[4];
+  int n = 0;
+  int at[kMaxSlots], val[kMaxSlots];
 };
 
 struct SaGraph {
@@ -332,7 +417,149 @@ struct SaGraph {
       p.t0 = (int)rng.below((uint32_t)lines);
       p.t1 = (int)rng.below((uint32_t)lines);
       p.r = (int)rng.below(2);
-    } while (!load(p));
+    } while (!load(p) || !sym_lines_ok(p));
+  }
+
+  // ---- Symmetric graphs (groups > 1): rotk[i*n + v] = v rotated i times,
+  // rotline[i*lines + t] = the line holding line t's edge rotated i times.
+  int groups = 1;
+  std::vector<int32_t> rotk, rotline;
+  // General graphs rotate by v -> v + n/groups (mod n); grids of the given
+  // height by ROTATE (grid/common.c:63-86): 180 degrees for 2 groups, 90 per
+  // copy for 4.  Fails unless the edge set is invariant under the rotation.
+  int sym_setup(int g_, int height) {
+    groups = g_;
+    const int B = n / groups;
+    rotk.assign((size_t)groups * n, 0);
+    const int width = height > 0 ? n / height : 0;
+    auto grid_rot = [&](int v, int deg) {
+      const int w = v / height, h = v % height;
+      if (groups == 2) return (width - w - 1) * height + (height - h - 1);
+      if (deg == 90) return h * height + (height - w - 1);
+      if (deg == 180) return (height - w - 1) * height + (height - h - 1);
+      return (height - h - 1) * height + w;
+    };
+    for (int q = 0; q < groups; ++q)
+      for (int v = 0; v < n; ++v)
+        rotk[(size_t)q * n + v] = q == 0 ? v : height > 0 ? grid_rot(v, 90 * q) : (v + q * B) % n;
+    std::unordered_map<uint64_t, int> line_of;
+    line_of.reserve((size_t)lines * 2);
+    for (int t = 0; t < lines; ++t) line_of[ekey(edges[2 * t], edges[2 * t + 1])] = t;
+    rotline.assign((size_t)groups * lines, 0);
+    for (int q = 0; q < groups; ++q)
+      for (int t = 0; t < lines; ++t) {
+        const int32_t *rk = rotk.data() + (size_t)q * n;
+        auto it = line_of.find(ekey(rk[edges[2 * t]], rk[edges[2 * t + 1]]));
+        if (it == line_of.end())
+          return set_error(GG_ERR_INVALID, "the graph is not invariant under its %d-fold rotation (line %d)",
+                           groups, t);
+        rotline[(size_t)q * lines + t] = it->second;
+      }
+    return GG_OK;
+  }
+  int rot(int q, int v) const { return rotk[(size_t)q * n + v]; }
+  int rline(int q, int t) const { return rotline[(size_t)q * lines + t]; }
+  // Lines in different orbits, neither mapped to itself by a rotation.
+  bool sym_lines_ok(const Prop &p) const {
+    for (int q = 1; q < groups; ++q)
+      if (rline(q, p.t0) == p.t0 || rline(q, p.t1) == p.t1 || rline(q, p.t0) == p.t1) return false;
+    return true;
+  }
+  // The vertices a move touches (4 per rotated copy).
+  int move_vertices(const Prop &p, int *out) const {
+    if (groups == 1) {
+      out[0] = p.a;
+      out[1] = p.b;
+      out[2] = p.c;
+      out[3] = p.d;
+      return 4;
+    }
+    for (int q = 0; q < groups; ++q) {
+      out[4 * q] = rot(q, p.a);
+      out[4 * q + 1] = rot(q, p.b);
+      out[4 * q + 2] = rot(q, p.c);
+      out[4 * q + 3] = rot(q, p.d);
+    }
+    return 4 * groups;
+  }
+  // The move's added edges, copy q: (nu[2q], nv[2q]), (nu[2q+1], nv[2q+1]).
+  void sym_added(const Prop &p, int *nu, int *nv) const {
+    const int q1 = p.r ? p.c : p.d, p2 = p.r ? p.b : p.c, q2 = p.r ? p.d : p.b;
+    for (int q = 0; q < groups; ++q) {
+      nu[2 * q] = rot(q, p.a);
+      nv[2 * q] = rot(q, q1);
+      nu[2 * q + 1] = rot(q, p2);
+      nv[2 * q + 1] = rot(q, q2);
+    }
+  }
+  bool sym_fresh(const Prop &p) const {
+    if (!sym_lines_ok(p)) return false;
+    int nu[2 * kMaxGroups], nv[2 * kMaxGroups];
+    sym_added(p, nu, nv);
+    if (!short_enough(nu[0], nv[0]) || !short_enough(nu[1], nv[1])) return false;
+    for (int h = 0; h < 2 * groups; ++h) {
+      if (nu[h] == nv[h] || set.has(ekey(nu[h], nv[h]))) return false;
+      for (int m = h + 1; m < 2 * groups; ++m)
+        if (ekey(nu[h], nv[h]) == ekey(nu[m], nv[m])) return false;
+    }
+    return true;
+  }
+  // The 4g adjacency entries: per copy q, a_q's b_q, b_q's a_q, c_q's d_q,
+  // d_q's c_q (the rows of distinct copies may be the same vertex; their
+  // entries differ since the removed edges do).
+  bool sym_slots(const Prop &p, Slots &s) const {
+    s.n = 0;
+    for (int q = 0; q < groups; ++q) {
+      const int a = rot(q, p.a), b = rot(q, p.b), c = rot(q, p.c), d_ = rot(q, p.d);
+      const int x0 = find(a, b), x1 = find(b, a), x2 = find(c, d_), x3 = find(d_, c);
+      if (x0 < 0 || x1 < 0 || x2 < 0 || x3 < 0) return false;
+      const int at[4] = {a * d + x0, b * d + x1, c * d + x2, d_ * d + x3};
+      const int val[4] = {p.r ? c : d_, p.r ? d_ : c, p.r ? a : b, p.r ? b : a};
+      for (int k = 0; k < 4; ++k) {
+        s.at[s.n] = at[k];
+        s.val[s.n] = val[k];
+        ++s.n;
+      }
+    }
+    return true;
+  }
+  // Lines written by a move (line, u, v): rotated copies of t0's and t1's
+  // new edges, oriented as the rotation of copy 0.
+  int sym_lines(const Prop &p, int32_t *out) const {
+    int nu[2 * kMaxGroups], nv[2 * kMaxGroups];
+    sym_added(p, nu, nv);
+    for (int q = 0; q < groups; ++q) {
+      const int l0 = rline(q, p.t0), l1 = rline(q, p.t1);
+      out[6 * q] = l0;
+      out[6 * q + 1] = nu[2 * q];
+      out[6 * q + 2] = nv[2 * q];
+      out[6 * q + 3] = l1;
+      out[6 * q + 4] = nu[2 * q + 1];
+      out[6 * q + 5] = nv[2 * q + 1];
+    }
+    return 2 * groups;
+  }
+  void sym_apply(const Prop &p, const Slots &s) {
+    for (int k = 0; k < s.n; ++k) adj[s.at[k]] = s.val[k];
+    if (!lineat.empty())
+      for (int q = 0; q < groups; ++q) {
+        const int l0 = rline(q, p.t0), l1 = rline(q, p.t1);
+        lineat[s.at[4 * q]] = l0;
+        lineat[s.at[4 * q + 1]] = l1;
+        lineat[s.at[4 * q + 2]] = p.r ? l0 : l1;
+        lineat[s.at[4 * q + 3]] = p.r ? l1 : l0;
+      }
+    int32_t upd[6 * kMaxGroups];
+    sym_lines(p, upd);
+    for (int q = 0; q < groups; ++q) {
+      for (int t : {rline(q, p.t0), rline(q, p.t1)}) set.remove(ekey(edges[2 * t], edges[2 * t + 1]));
+    }
+    for (int h = 0; h < 2 * groups; ++h) {
+      const int t = upd[3 * h];
+      edges[2 * t] = upd[3 * h + 1];
+      edges[2 * t + 1] = upd[3 * h + 2];
+      set.add(ekey(upd[3 * h + 1], upd[3 * h + 2]));
+    }
   }
   // Grid graphs: line t1 is drawn among the lines at a vertex within `length`
   // steps of line t0's first endpoint, so that most draws respect the limit.
@@ -364,7 +591,7 @@ struct SaGraph {
     const int c = cx * grid_height + cy;
     p.t1 = lineat[(size_t)c * d + rng.below((uint32_t)d)];
     p.r = (int)rng.below(2);
-    return load(p);
+    return load(p) && sym_lines_ok(p);
   }
   // A valid move (simple graph kept, grid lengths respected); false after
   // `tries` failed draws.
@@ -386,6 +613,7 @@ struct SaGraph {
   // True when neither added edge exists yet (the move keeps the graph simple)
   // and, on a grid, both respect the length limit.
   bool fresh(const Prop &p) const {
+    if (groups > 1) return sym_fresh(p);
     const int q1 = p.r ? p.c : p.d, p2 = p.r ? p.b : p.c, q2 = p.r ? p.d : p.b;
     return short_enough(p.a, q1) && short_enough(p2, q2) && !set.has(ekey(p.a, q1)) &&
            !set.has(ekey(p2, q2));
@@ -397,8 +625,10 @@ struct SaGraph {
   }
   // Adjacency entries the move rewrites: a's b, b's a, c's d, d's c.
   bool slots(const Prop &p, Slots &s) const {
+    if (groups > 1) return sym_slots(p, s);
     const int x0 = find(p.a, p.b), x1 = find(p.b, p.a), x2 = find(p.c, p.d), x3 = find(p.d, p.c);
     if (x0 < 0 || x1 < 0 || x2 < 0 || x3 < 0) return false;
+    s.n = 4;
     s.at[0] = p.a * d + x0;
     s.at[1] = p.b * d + x1;
     s.at[2] = p.c * d + x2;
@@ -410,6 +640,10 @@ struct SaGraph {
     return true;
   }
   void apply(const Prop &p, const Slots &s) {
+    if (groups > 1) {
+      sym_apply(p, s);
+      return;
+    }
     for (int k = 0; k < 4; ++k) adj[s.at[k]] = s.val[k];
     if (!lineat.empty()) {  // a's and d's (r=0) or a's and c's (r=1) new edges are line t0
       lineat[s.at[0]] = p.t0;
@@ -544,6 +778,26 @@ extern "C" int gg_apply_move(int lines, int32_t *edges, int line0, int line1, in
   return GG_OK;
 }
 
+extern "C" int gg_apply_move_sym(int nodes, int degree, int groups, int grid_height, int32_t *edges,
+                                 int line0, int line1, int variant) {
+  if (!edges || nodes < 4 || degree < 1) return set_error(GG_ERR_INVALID, "bad move arguments");
+  const int lines = (int)((int64_t)nodes * degree / 2);
+  if (groups <= 1) return gg_apply_move(lines, edges, line0, line1, variant);
+  if (groups > kMaxGroups || nodes % groups || line0 < 0 || line1 < 0 || line0 >= lines ||
+      line1 >= lines || (variant != 0 && variant != 1))
+    return set_error(GG_ERR_INVALID, "bad symmetric move");
+  SaGraph g(nodes, degree, edges);
+  int rc = g.init();
+  if (!rc) rc = g.sym_setup(groups, grid_height > 0 ? grid_height : 0);
+  if (rc) return rc;
+  Prop p{line0, line1, variant, 0, 0, 0, 0};
+  Slots s;
+  if (!g.load(p) || !g.sym_lines_ok(p) || !g.slots(p, s))
+    return set_error(GG_ERR_INVALID, "lines %d, %d do not make a symmetric move", line0, line1);
+  g.apply(p, s);
+  return GG_OK;
+}
+
 // The device proposal draw of ranked steps, for tests: `count` proposals of
 // the graph (edges [lines][2], adjacency [n][d]) for (seed, step).
 extern "C" int gg_sa_proposals(int nodes, int degree, const int32_t *edges, const int32_t *adj,
@@ -634,13 +888,22 @@ extern "C" int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa
   const int E = std::max(1, P->exact);
   const bool use_sur = P->use_surrogate != 0;
   const int NP = use_sur ? std::max(P->proposals, 1) : 0;
-  const int npmax = std::max(NP, E);
+  const int npmax = std::max(NP, use_sur ? std::max(E, std::max(64, 8 * E)) : E);
   const double t0 = now_s();
   double t_exact = 0, t_sur = 0, t_feat = 0, t_train = 0;
   int rc = gg_init(-1);
   if (rc) return rc;
   SaGraph g(nodes, degree, edges);
   if ((rc = g.init())) return rc;
+  const int groups = std::max(1, P->groups);
+  if (groups > kMaxGroups || nodes % groups || (nodes * (int64_t)degree / 2) % groups)
+    return set_error(GG_ERR_INVALID, "groups %d must divide nodes %d and lines (at most %d)", groups,
+                     nodes, kMaxGroups);
+  if (groups > 1 && P->grid_height > 0 && groups != 2 && groups != 4)
+    return set_error(GG_ERR_INVALID, "grid graphs take 2 or 4 groups");
+  if (groups > 1 && P->incremental)
+    return set_error(GG_ERR_INVALID, "the incremental scorer takes groups = 1");
+  if (groups > 1 && (rc = g.sym_setup(groups, P->grid_height > 0 ? P->grid_height : 0))) return rc;
   if (P->grid_height > 0) {
     if (nodes % P->grid_height || P->grid_length < 1)
       return set_error(GG_ERR_INVALID, "grid height %d / length %d do not fit %d nodes", P->grid_height,
@@ -679,11 +942,27 @@ extern "C" int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa
     add[3] = p.r ? p.d : p.b;
   };
 
-  gg_graph shape = {nodes, degree, nodes, 1, 0, GG_GRAPH_SYMMETRIC, g.adj.data(), nullptr};
+  // Symmetric graphs are scored from their base sources with the groups
+  // weights; a grid's base quadrant goes first through its rotate_hash
+  // (create_rotate_hash, grid/common.c:312-341).
+  std::vector<int32_t> rhash;
+  if (groups > 1 && g.grid_height > 0) {
+    const int h = g.grid_height, w = nodes / h, B = nodes / groups, bh = h / 2;
+    rhash.assign(nodes, -1);
+    for (int i = 0; i < B; ++i) {
+      const int j = (i / bh) * h + (i % bh);
+      rhash[j] = i;
+      for (int q = 1; q < groups; ++q) rhash[g.rot(q, j)] = i + q * B;
+    }
+    for (int v = 0; v < nodes; ++v)
+      if (rhash[v] < 0) return set_error(GG_ERR_INVALID, "grid %dx%d has no %d-fold base quadrant", w, h, groups);
+  }
+  gg_graph shape = {nodes, degree, nodes / groups, groups, 0, GG_GRAPH_SYMMETRIC, g.adj.data(),
+                    rhash.empty() ? nullptr : rhash.data()};
+  const int max_patch = 4 * groups * P->combine;
   PlanHolder plan;
-  plan.p = gg_plan_create_patched(E, &shape, 4 * P->combine);
-  if (!plan.p) return GG_ERR_HIP;
-  const int max_patch = 4 * P->combine;
+  plan.p = gg_plan_create_patched(E, &shape, max_patch);
+  if (!plan.p) return gg_last_error_code() ? gg_last_error_code() : GG_ERR_HIP;
   std::vector<gg_result> res(E);
   std::vector<int32_t> patches((size_t)E * max_patch * 2, -1);
   rc = gg_plan_set_base(plan.p, g.adj.data());
@@ -704,7 +983,14 @@ extern "C" int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa
   vs.maxd = std::min(std::max(P->hist_maxd, 2), 15);
   const int cap = 16384;
   const int bt = std::max(P->batch_train, 32);
-  DevArr<int32_t> props_d, idx_d, tidx_d, edges_d, plines_d, upd_d, lineat_d, offsets_d;
+  DevArr<int32_t> props_d, idx_d, tidx_d, edges_d, plines_d, upd_d, lineat_d, offsets_d, rotk_d,
+      rotline_d;
+  // Lines (2 per copy) and grid line-table entries (4 per copy) of an
+  // accepted candidate's moves.
+  const size_t upd_rows = (size_t)6 * groups * kMaxCombine;
+  // Explore steps of surrogate runs draw on the device too: this many
+  // proposals, of which the first valid ones become the singles.
+  const int nexp = std::max(64, 8 * E);
   struct Pinned {
     int32_t *props = nullptr, *lines = nullptr, *upd = nullptr;
     ~Pinned() {
@@ -727,12 +1013,19 @@ extern "C" int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa
         (rc = vstat_d.alloc((size_t)nodes * kVStat)) ||
         (rc = idx_d.alloc(2 * (size_t)std::max(npmax, bt))) || (rc = tidx_d.alloc((size_t)bt)) ||
         (rc = edges_d.alloc(2 * (size_t)g.lines)) || (rc = plines_d.alloc((size_t)npmax * 3)) ||
-        (rc = upd_d.alloc(3 * 6 * (size_t)kMaxCombine)))
+        (rc = upd_d.alloc(3 * upd_rows)))
       return rc;
     if (hipHostMalloc((void **)&pin.props, (size_t)npmax * 5 * 4) != hipSuccess ||
         hipHostMalloc((void **)&pin.lines, (size_t)npmax * 3 * 4) != hipSuccess ||
-        hipHostMalloc((void **)&pin.upd, 3 * 6 * (size_t)kMaxCombine * 4) != hipSuccess)
+        hipHostMalloc((void **)&pin.upd, 3 * upd_rows * 4) != hipSuccess)
       return set_error(GG_ERR_HIP, "pinned allocation failed");
+    if (groups > 1) {  // the rotation tables of the symmetric draws
+      if ((rc = rotk_d.alloc(g.rotk.size())) || (rc = rotline_d.alloc(g.rotline.size()))) return rc;
+      if (hipMemcpy(rotk_d.p, g.rotk.data(), g.rotk.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+          hipMemcpy(rotline_d.p, g.rotline.data(), g.rotline.size() * 4, hipMemcpyHostToDevice) !=
+              hipSuccess)
+        return set_error(GG_ERR_HIP, "rotation table upload failed");
+    }
     if (hipMemcpy(edges_d.p, edges, 2 * (size_t)g.lines * 4, hipMemcpyHostToDevice) != hipSuccess)
       return set_error(GG_ERR_HIP, "edge list upload failed");
     if (g.grid_height > 0) {  // the line table and length ball of the grid draws
@@ -774,9 +1067,11 @@ extern "C" int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa
                          (P->explore_every > 0 && (step - P->warmup) % P->explore_every == P->explore_every - 1);
     // --- proposals
     double tp = now_s();
-    const int np = explore ? E : NP;
-    // Ranked steps draw their proposals on the device.
-    const bool dev_props = use_sur && !explore;
+    // Surrogate runs draw every proposal on the device (ranked steps: NP to
+    // rank; explore steps: nexp, the first valid ones becoming the singles);
+    // plain runs draw their singles on the host.
+    const bool dev_props = use_sur;
+    const int np = use_sur ? (explore ? nexp : NP) : E;
     for (int i = 0; i < (dev_props ? 0 : np); ++i) {
       Prop &p = props[i];
       if (explore) {
@@ -802,7 +1097,15 @@ extern "C" int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa
     // --- features + ranking on the device
     if (use_sur) {
       const double ts = now_s();
-      if (dev_props && g.grid_height > 0) {
+      if (dev_props && groups > 1) {
+        hipLaunchKernelGGL(gg_bsa_propose_sym, dim3((np + 255) / 256), dim3(256), 0, es, np, P->seed,
+                           (uint64_t)step, g.lines, (const int32_t *)edges_d.p,
+                           (const int32_t *)gg_plan_base_device(plan.p), degree, nodes, groups,
+                           (const int32_t *)rotk_d.p, (const int32_t *)rotline_d.p,
+                           (const int32_t *)lineat_d.p, g.grid_height, g.grid_length,
+                           (const int32_t *)offsets_d.p, (int)g.offsets.size(), props_d.p, plines_d.p);
+        BSA_HIP(hipGetLastError());
+      } else if (dev_props && g.grid_height > 0) {
         hipLaunchKernelGGL(gg_bsa_propose_grid, dim3((np + 255) / 256), dim3(256), 0, es, np,
                            P->seed, (uint64_t)step, g.lines, (const int32_t *)edges_d.p,
                            (const int32_t *)gg_plan_base_device(plan.p), (const int32_t *)lineat_d.p,
@@ -851,6 +1154,18 @@ extern "C" int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa
       for (int i = 0; i < np; ++i) noise[i] = -1.0f - (float)rng.below(1u << 24) * (1.0f / (1u << 24));
     // --- candidates
     int ncand = explore ? E : 0;
+    if (explore && dev_props) {
+      // The first valid device draws become this step's singles.
+      ncand = 0;
+      for (int i = 0; i < np && ncand < E; ++i) {
+        Slots sl;
+        if (pred[i] >= 1e29f || !g.fresh(props[i]) || !g.slots(props[i], sl)) continue;
+        cand[ncand].nm = 1;
+        cand[ncand].prop[0] = i;
+        cand[ncand].sl[0] = sl;
+        ++ncand;
+      }
+    }
     if (!explore) {
       const double tc = now_s();
       std::iota(order.begin(), order.begin() + np, 0);
@@ -870,15 +1185,18 @@ extern "C" int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa
           cand_verts.clear();
           if (++c >= E) break;
         }
+        int mv[4 * kMaxGroups];
+        const int nmv = g.move_vertices(p, mv);
         bool clash = false;
-        for (int v : cand_verts) clash |= v == p.a || v == p.b || v == p.c || v == p.d;
+        for (int v : cand_verts)
+          for (int k = 0; k < nmv; ++k) clash |= v == mv[k];
         Slots s;
         if (clash || !g.fresh(p) || !g.slots(p, s)) continue;
         Cand &C = cand[c];
         C.prop[C.nm] = i;
         C.sl[C.nm] = s;
         ++C.nm;
-        cand_verts.insert(cand_verts.end(), {p.a, p.b, p.c, p.d});
+        cand_verts.insert(cand_verts.end(), mv, mv + nmv);
         if (C.nm >= m_cur) {
           cand_verts.clear();
           ++c;
@@ -889,12 +1207,14 @@ extern "C" int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa
     }
     // --- exact scoring of all candidates in one plan launch
     std::fill(patches.begin(), patches.end(), -1);
-    for (int c = 0; c < ncand; ++c)
+    for (int c = 0; c < ncand; ++c) {
+      size_t at = (size_t)c * max_patch;
       for (int k = 0; k < cand[c].nm; ++k)
-        for (int j = 0; j < 4; ++j) {
-          patches[((size_t)c * max_patch + 4 * k + j) * 2] = cand[c].sl[k].at[j];
-          patches[((size_t)c * max_patch + 4 * k + j) * 2 + 1] = cand[c].sl[k].val[j];
+        for (int j = 0; j < cand[c].sl[k].n; ++j, ++at) {
+          patches[at * 2] = cand[c].sl[k].at[j];
+          patches[at * 2 + 1] = cand[c].sl[k].val[j];
         }
+    }
     const double te = now_s();
     if (incr.p && !explore) {  // the first ranked step retires the scorer
       gg_incr_destroy(incr.p);
@@ -1002,7 +1322,8 @@ extern "C" int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa
       } else if (nS <= cur_S) {
         acc = true;
       } else {
-        const double diff = -((double)nS - (double)cur_S);
+        // general/sa.c:395: (ASPL - ASPL') n (n-1) / groups = -dS / groups
+        const double diff = -((double)nS - (double)cur_S) / groups;
         acc = std::exp(diff / temp) > (rng.below(1u << 30) + 1.0) / ((double)(1u << 30) + 2.0);
       }
       improved = nd < cur_diam || (nd == cur_diam && nS < cur_S);
@@ -1026,19 +1347,20 @@ extern "C" int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa
           int nu = 0;
           for (int k = 0; k < C.nm; ++k) {
             const Prop &p = props[C.prop[k]];
-            for (int t : {p.t0, p.t1}) {
-              pin.upd[3 * nu] = t;
-              pin.upd[3 * nu + 1] = edges[2 * t];
-              pin.upd[3 * nu + 2] = edges[2 * t + 1];
-              ++nu;
-            }
+            for (int q = 0; q < groups; ++q)
+              for (int t : {groups > 1 ? g.rline(q, p.t0) : p.t0, groups > 1 ? g.rline(q, p.t1) : p.t1}) {
+                pin.upd[3 * nu] = t;
+                pin.upd[3 * nu + 1] = edges[2 * t];
+                pin.upd[3 * nu + 2] = edges[2 * t + 1];
+                ++nu;
+              }
           }
           // Grid graphs: the line table entries the moves rewrote, as
           // (-1 - slot, line, 0) records after the lines.
           int nl = 0;
           if (g.grid_height > 0)
             for (int k = 0; k < C.nm; ++k)
-              for (int j = 0; j < 4; ++j) {
+              for (int j = 0; j < C.sl[k].n; ++j) {
                 const int at = C.sl[k].at[j];
                 pin.upd[3 * (nu + nl)] = -1 - at;
                 pin.upd[3 * (nu + nl) + 1] = g.lineat[at];

@@ -2035,7 +2035,7 @@ static gg_plan *plan_fail(gg_plan *p) {
 
 static gg_plan *plan_create(int count, const gg_graph *shapes, int max_patch) {
   if (gg::init(-1)) return nullptr;
-  if (count <= 0 || !shapes || max_patch < 0 || max_patch > 512) {
+  if (count <= 0 || !shapes || max_patch < 0 || max_patch > 4096) {
     set_error(GG_ERR_INVALID, "bad plan arguments");
     return nullptr;
   }

@@ -252,7 +252,7 @@ class BsaParams(ctypes.Structure):
                 ("lr", ctypes.c_float), ("hist_maxd", ctypes.c_int32),
                 ("feature_refresh", ctypes.c_int32), ("log_capacity", ctypes.c_int32),
                 ("grid_height", ctypes.c_int32), ("grid_length", ctypes.c_int32),
-                ("incremental", ctypes.c_int32), ("pad", ctypes.c_int32)]
+                ("incremental", ctypes.c_int32), ("groups", ctypes.c_int32)]
 
 
 class BsaStats(ctypes.Structure):
@@ -285,11 +285,13 @@ def sa_batched(edges, nodes: int, degree: int, *, max_steps: int = 100, time_bud
                train_steps: int = 1, batch_train: int = 256, lr: float = 1e-3, hist_maxd: int = 8,
                feature_refresh: int = 100, log_capacity: int = 100000,
                accepted_capacity: int = 100000, grid_height: int = 0,
-               grid_length: int = 0, incremental: bool = False) -> BatchedResult:
+               grid_length: int = 0, incremental: bool = False, groups: int = 1) -> BatchedResult:
     """Batched (and, with use_surrogate, MLP-ranked) SA on the GPU; see
     include/gg_host.h.  exact = 1 without the surrogate is plain single-graph
     GPU SA with a device-resident adjacency.  grid_height > 0 keeps every
-    edge of a (groups = 1) grid graph within grid_length Manhattan steps."""
+    edge of a grid graph within grid_length Manhattan steps.  groups > 1: a
+    rotationally symmetric graph (general: v -> v + n/groups; grid: 2 or 4
+    groups), moved by 2g-edge 2-opts that keep the symmetry."""
     lib = L.lib()
     if not getattr(lib, "_bsa_sigs", False):
         lib.gg_sa_batched.argtypes = [ctypes.c_int, ctypes.c_int, L.i32p,
@@ -306,7 +308,7 @@ def sa_batched(edges, nodes: int, degree: int, *, max_steps: int = 100, time_bud
     p = BsaParams(max_steps, time_budget_s, target_aspl, proposals, exact, combine, warmup,
                   explore_every, int(use_surrogate), int(random_ranking), max_temp, min_temp,
                   seed, train_steps, batch_train, lr, hist_maxd, feature_refresh, log_capacity,
-                  grid_height, grid_length, int(incremental))
+                  grid_height, grid_length, int(incremental), int(groups))
     st = BsaStats()
     lt = np.zeros(max(log_capacity, 1))
     la = np.zeros(max(log_capacity, 1))
@@ -325,8 +327,17 @@ def sa_batched(edges, nodes: int, degree: int, *, max_steps: int = 100, time_bud
     return BatchedResult(e, stats, lt[:ns], la[:ns], ld[:ns], acc[:na], ad[:na], asum[:na])
 
 
-def apply_move(edges: np.ndarray, line0: int, line1: int, variant: int) -> None:
-    """Applies a logged batched-SA move to `edges` in place."""
+def apply_move(edges: np.ndarray, line0: int, line1: int, variant: int, *, nodes: int = 0,
+               degree: int = 0, groups: int = 1, grid_height: int = 0) -> None:
+    """Applies a logged batched-SA move to `edges` in place (groups > 1: the
+    move and its rotated copies; nodes and degree then required)."""
+    if groups > 1:
+        lib = L.lib()
+        lib.gg_apply_move_sym.argtypes = [ctypes.c_int] * 4 + [L.i32p] + [ctypes.c_int] * 3
+        lib.gg_apply_move_sym.restype = ctypes.c_int
+        L.check(lib.gg_apply_move_sym(nodes, degree, groups, grid_height, L.ptr(edges), int(line0),
+                                      int(line1), int(variant)))
+        return
     L.check(L.lib().gg_apply_move(edges.shape[0], L.ptr(edges), int(line0), int(line1),
                                   int(variant)))
 

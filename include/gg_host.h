@@ -260,7 +260,11 @@ typedef struct gg_bsa_params {
                              exact results, faster at large n.  Without the surrogate:
                              every step; with it: the explore steps up to the first
                              ranked step (merged candidates flag nearly every source) */
-  int32_t pad;
+  int32_t groups;         /* > 1: a rotationally symmetric graph (general: v -> v + n/groups;
+                             grid: 2 or 4 groups, ROTATE of grid/common.c:63-86); every move
+                             is the 2g-edge 2-opt of exchange_edge_2opt (general/sa.c:217-369,
+                             grid/sa.c:137-221) on two lines and their rotated copies, drawn on
+                             the device; scored from the base sources.  0 or 1: none */
 } gg_bsa_params;
 
 typedef struct gg_bsa_stats {
@@ -311,6 +315,11 @@ int gg_sa_proposals(int nodes, int degree, const int32_t *edges, const int32_t *
 /* Applies one logged move (line0, line1, variant) to an edge list the way
  * gg_sa_batched does (for replays). */
 int gg_apply_move(int lines, int32_t *edges, int line0, int line1, int variant);
+/* The same for a symmetric graph's move (gg_bsa_params.groups > 1): lines
+ * line0, line1 and their rotated copies re-paired with `variant`; the edge
+ * list must be invariant under the rotation (grid_height > 0: a grid). */
+int gg_apply_move_sym(int nodes, int degree, int groups, int grid_height, int32_t *edges, int line0,
+                      int line1, int variant);
 
 #ifdef __cplusplus
 }

@@ -283,3 +283,101 @@ def test_gpu_surrogate_sa_incremental_warmup_is_the_same_run():
     assert np.array_equal(full.accepted_diam, inc.accepted_diam)
     assert np.array_equal(full.accepted_sum, inc.accepted_sum)
     assert np.array_equal(full.edges, inc.edges)
+
+
+def _rot_tables(n, g, h=0):
+    """Vertex rotations of a symmetric graph: general v -> v + n/g, grids
+    ROTATE (graphs.rotate)."""
+    if not h:
+        return [(np.arange(n) + q * (n // g)) % n for q in range(g)]
+    w = n // h
+    return [np.arange(n)] + [np.array([G.rotate(v, h, w, g, 90 * q) for v in range(n)])
+                             for q in range(1, g)]
+
+
+def _check_symmetric_graph(e, n, d, g, h=0, r=0):
+    keys = np.sort(e, axis=1)
+    assert np.all(keys[:, 0] != keys[:, 1])
+    k = keys[:, 0].astype(np.int64) * n + keys[:, 1]
+    assert len(np.unique(k)) == len(k)
+    assert np.all(np.bincount(e.ravel(), minlength=n) == d)
+    ks = set(k.tolist())
+    for rot in _rot_tables(n, g, h)[1:]:
+        re = np.sort(rot[e], axis=1)
+        assert set((re[:, 0].astype(np.int64) * n + re[:, 1]).tolist()) == ks
+    if h:
+        x, y = e // h, e % h
+        assert np.all(np.abs(x[:, 0] - x[:, 1]) + np.abs(y[:, 0] - y[:, 1]) <= r)
+
+
+def _block_layout(e, g):
+    """The reference's line layout (copies of line t at t + i*lines/g) from
+    the generator's interleaved one (copies at consecutive lines)."""
+    L = len(e)
+    out = np.empty_like(e)
+    for orbit in range(L // g):
+        for i in range(g):
+            out[i * (L // g) + orbit] = e[orbit * g + i]
+    return out
+
+
+@pytest.mark.parametrize("kind,g,surrogate", [("general", 2, True), ("general", 4, True),
+                                              ("general", 4, False), ("grid", 2, True),
+                                              ("grid", 4, True), ("grid", 4, False)])
+def test_gpu_batched_symmetric_sa_keeps_symmetry_and_is_exact(kind, g, surrogate):
+    """Symmetric graphs in the batched SA (2g-edge moves drawn on the device):
+    every accepted graph, replayed from the log, stays simple, regular,
+    rotation invariant (and within the grid length), and its (diameter, S)
+    equals the oracle's groups-weighted evaluation."""
+    if kind == "general":
+        n, d, h, r = 1024, 8, 0, 0
+        e0 = G.random_symmetric(n, d, g, 5)
+        if g == 4:
+            e0 = _block_layout(e0, g)
+        kw = {}
+    else:
+        w = h = 12
+        n, d, r = w * h, 4, 3
+        e0 = G.random_grid_symmetric(w, h, d, r, g, 5)
+        kw = dict(grid_height=h, grid_length=r)
+    print(f"symmetric batched sa {kind} g={g} surrogate={surrogate}", flush=True)
+    _check_symmetric_graph(e0, n, d, g, h, r)
+    res = SA.sa_batched(e0, n, d, groups=g, max_steps=60, proposals=512, exact=4, combine=4,
+                        warmup=10, explore_every=4, use_surrogate=surrogate, seed=3, train_steps=2,
+                        batch_train=128, feature_refresh=5, max_temp=10.0, min_temp=0.5, **kw)
+    assert res.stats["accepts"] > 0
+    rh = G.rotate_hash(n, h, n // h, g) if h else None
+    e = e0.copy()
+    checked = 0
+    for k, (t0, t1, v) in enumerate(res.accepted):
+        SA.apply_move(e, t0, t1, v, nodes=n, degree=d, groups=g, grid_height=h)
+        if res.accepted_diam[k] < 0:
+            continue
+        _check_symmetric_graph(e, n, d, g, h, r)
+        adj = G.edges_to_adj(n, d, e)
+        if h:
+            o = oracle.grid_evaluation(adj, g, n // g, h, h // 2, rh)
+        else:
+            o = oracle.matrix_op(adj, n // g, g)
+        assert (o.diameter, o.S) == (res.accepted_diam[k], int(res.accepted_sum[k])), k
+        checked += 1
+    assert checked > 0
+    adj = G.edges_to_adj(n, d, res.edges)
+    o = oracle.grid_evaluation(adj, g, n // g, h, h // 2, rh) if h else oracle.matrix_op(adj, n // g, g)
+    assert o.diameter == res.stats["best_diameter"] and o.aspl == res.stats["best_aspl"]
+
+
+def test_gpu_symmetric_device_proposals_keep_symmetry():
+    """The symmetric device draw at C2's shape (n=4096, d=16, g=4): a valid
+    proposal applied with its rotated copies keeps the graph simple,
+    regular and rotation invariant; the same (seed, step) draws the same."""
+    n, d, g = 4096, 16, 4
+    e0 = G.random_symmetric(n, d, g, 11)
+    res = SA.sa_batched(e0, n, d, groups=g, max_steps=8, proposals=4096, exact=1, combine=8,
+                        warmup=2, explore_every=0, use_surrogate=True, seed=9)
+    e = e0.copy()
+    for (t0, t1, v), dm in zip(res.accepted, res.accepted_diam):
+        SA.apply_move(e, t0, t1, v, nodes=n, degree=d, groups=g)
+    _check_symmetric_graph(e, n, d, g)
+    o = oracle.matrix_op(G.edges_to_adj(n, d, res.edges), n // g, g)
+    assert o.diameter == res.stats["best_diameter"] and o.aspl == res.stats["best_aspl"]
