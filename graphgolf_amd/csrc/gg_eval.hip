@@ -955,7 +955,6 @@ Config config_for(int n, int d, int b, int g, int c, bool batched) {
   if (E.forced_bits) cfg.bits = E.forced_bits;
   if (cfg.bits < 8 || cfg.bits > maxb) cfg.bits = maxb;
   if (cfg.block == 256 && n > 2048) cfg.block = 0;
-  if (!batched) cfg.persistent = 0;
   return cfg;
 }
 
@@ -1031,7 +1030,7 @@ int assemble(int count, const gg_graph *gs, int hist_maxd, Assembly &A) {
   const Config bc = config_for(gs[big].nodes, gs[big].degree, gs[big].based_nodes, gs[big].groups,
                                gs[big].added_centers, batched);
   A.cfg = bc;
-  A.persistent = batched && bc.persistent && hist_maxd == 0;
+  A.persistent = bc.persistent && hist_maxd == 0;
   A.order_mode = effective_order(bc);
   const int n_max = gs[big].nodes;
   for (int i = 0; i < count; ++i) {
@@ -1845,8 +1844,8 @@ static gg_plan *plan_create(int count, const gg_graph *shapes, int max_patch);
 // given) through a plan and keeps the fastest in the tuning table under the
 // shape bucket of the largest graph.  Candidates: every word width that
 // fits, 256- or 1024-thread blocks (up to 2048 vertices), row order off or
-// on, and for batches one queue-driven launch or one launch per kernel
-// instance.  Every candidate must return the same (connected, diameter, S)
+// on, and one queue-driven launch or one launch with a workgroup per unit
+// (per kernel instance, for batches).  Every candidate must return the same (connected, diameter, S)
 // for every graph, else the tuner fails with GG_ERR_STATE.
 static int tune_run(int count, const gg_graph *graphs, int reps, Config *best_cfg, double *best_ms) {
   if (count < 1 || !graphs || reps < 1) return set_error(GG_ERR_INVALID, "bad tuning arguments");
@@ -1868,10 +1867,7 @@ static int tune_run(int count, const gg_graph *graphs, int reps, Config *best_cf
     for (int block : {0, 1024}) {
       if (block == 1024 && n_max > 2048) continue;  // 1024 is the only choice there
       for (int order : {0, 2})
-        for (int pers : {1, 0}) {
-          if (pers && !batched) continue;
-          cands.push_back({bits, block, order, pers});
-        }
+        for (int pers : {1, 0}) cands.push_back({bits, block, order, pers});
     }
   }
   const bool saved_on = E.forced_cfg;

@@ -146,7 +146,8 @@ int gg_synchronize(void);
 
 /* ---- Tuning.  A launch configuration is (word bits 8/16/32/64, threads per
  * block 0 = from n / 256 / 1024, row-order mode 0 off / 1 by size / 2 on,
- * persistent: a batch in one queue-driven launch).  Configurations are kept
+ * persistent: one queue-driven launch whose workgroups take the (graph, tile)
+ * units from a device counter).  Configurations are kept
  * per shape bucket (ceil(log2 n) of the graph, or of a batch's largest graph;
  * degree; groups; centres or not; batch or single) and consulted by every
  * evaluation and plan made afterwards; without an entry the built-in choice

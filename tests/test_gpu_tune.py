@@ -50,7 +50,7 @@ def test_autotune_single_records_bucket_and_round_trips(tmp_path):
     bits, ms = E.autotune(4096, 16, 1024, 4, 1, 3)
     assert bits in (8, 16, 32, 64) and ms > 0
     cfg = E.tune_lookup(4096, 16, 1024, 4)
-    assert cfg["word_bits"] == bits and cfg["persistent"] == 0
+    assert cfg["word_bits"] == bits and cfg["persistent"] in (0, 1)
     # the same bucket covers other sizes of ceil(log2 n) = 12
     assert E.tune_lookup(3000, 16, 750, 4)["word_bits"] == min(bits, 64)
     path = str(tmp_path / "tune.txt")
