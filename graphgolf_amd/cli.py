@@ -14,8 +14,10 @@ Edge files are the reference's ("a b" lines; grid "x,y x,y" lines).  A run
 follows the reference program's random() stream from the same seed: the same
 start-graph moves (skipped with -M, which resumes from a whole graph written
 by -o), the same annealing moves, the same output file.  -B and -L choose the
-reference's evaluation algorithm; every algorithm gives the same results and
-this build always runs the device evaluator, so they only change the banner.
+reference's evaluation algorithm.  The graphs these programs evaluate are
+undirected and invariant under their rotation, where bfs() and matrix_op give
+the same diameter and ASPL (bit for bit), so the device evaluator serves all
+three with the same kernels and the flags change the banner only.
 -D / -d print the proposed temperatures as the reference does; -N lets
 repeated edges stay in the graph (the adjacency is rebuilt after each move).  -O (listed by the
 reference's help, absent from its option string) skips the graph checks.
