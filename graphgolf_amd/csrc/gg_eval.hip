@@ -1509,7 +1509,7 @@ int eval_host(int count, const gg_graph *gs, gg_result *out, bool device_ptrs, i
   DevBuf trace_buf;
   const int trace_units = std::min<int>(g_trace_units, (int)A.units.size());
   if (trace_units > 0) {
-    size_t tb = (size_t)A.units.size() * kTraceLevels * 2 * sizeof(uint64_t);
+    size_t tb = (size_t)A.units.size() * kTraceLevels * kTraceFields * sizeof(uint64_t);
     if ((rc = trace_buf.ensure(tb))) return rc;
     GG_HIP(hipMemsetAsync(trace_buf.p, 0, tb, E.stream));
   }
@@ -1541,7 +1541,7 @@ int eval_host(int count, const gg_graph *gs, gg_result *out, bool device_ptrs, i
     GG_HIP(hipMemcpyAsync(hist_host, hist_dev, hist_bytes, hipMemcpyDeviceToHost, E.stream));
   if (trace_units > 0)
     GG_HIP(hipMemcpyAsync(g_trace_host, trace_buf.p,
-                          (size_t)trace_units * kTraceLevels * 2 * sizeof(uint64_t),
+                          (size_t)trace_units * kTraceLevels * kTraceFields * sizeof(uint64_t),
                           hipMemcpyDeviceToHost, E.stream));
   rc = collect(A, D, out);
   trace_buf.release();

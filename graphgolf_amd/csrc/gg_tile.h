@@ -43,8 +43,11 @@ constexpr int kTileList = 2;    // sources from a list, persistent over *src_cou
 constexpr int kTileProbe = 3;   // sources from a list (UnitDesc units) + probe levels
 constexpr int kTileSrc = 4;     // kTileList + per-source distance sums and eccentricities
 
-// Per-level trace record: (cycles since the unit started, mode word).
+// Per-level trace record: cycles since the unit started at the level's end,
+// mode word, and (wave 0's view) cycles when its reads of the level were
+// issued and when the barrier after them released.
 constexpr int kTraceLevels = 64;
+constexpr int kTraceFields = 4;
 
 struct UnitDesc {
   int32_t graph;
@@ -89,6 +92,20 @@ template <typename W> __device__ __forceinline__ W lds_or(W *R, uint32_t x, W f)
     unsigned *w = (unsigned *)R + (x >> 2);
     unsigned sh = (x & 3u) * 8u;
     return (W)(atomicOr(w, (unsigned)f << sh) >> sh);
+  }
+}
+
+// LDS atomic OR of `f` into element x of R, without the old value (the
+// non-returning form: the wave does not wait for it).
+template <typename W> __device__ __forceinline__ void lds_or_nr(W *R, uint32_t x, W f) {
+  if constexpr (sizeof(W) == 8) {
+    atomicOr((unsigned long long *)(R + x), (unsigned long long)f);
+  } else if constexpr (sizeof(W) == 4) {
+    atomicOr((unsigned *)(R + x), (unsigned)f);
+  } else if constexpr (sizeof(W) == 2) {
+    atomicOr((unsigned *)R + (x >> 1), (unsigned)f << ((x & 1u) * 16u));
+  } else {
+    atomicOr((unsigned *)R + (x >> 2), (unsigned)f << ((x & 3u) * 8u));
   }
 }
 
@@ -151,13 +168,19 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
 
 // ---------------------------------------------------------------------------
 // The tile BFS.  One workgroup = one UnitDesc.  LDS layout (dynamic):
-//   R[n_cap] (W) | qb[2][qcap] (W) | qv[2][qcap] (u16) | red[2][16][4] |
+//   R[n_cap] (W) | qb[2][qcap] (W) | qv[2][qcap] (u16) | red[3][16][4] |
 //   qcount[4] | sacc[2][64]
 // tile_lds_bytes gives its size for `wb`-byte words.
 __host__ __device__ constexpr size_t tile_align16(size_t x) { return (x + 15) & ~(size_t)15; }
 __host__ __device__ constexpr size_t tile_lds_bytes(size_t wb, int n_cap, int qcap) {
   return tile_align16((size_t)n_cap * wb) + tile_align16((size_t)2 * qcap * wb) +
-         tile_align16((size_t)2 * qcap * 2) + 2 * 16 * 4 * 4 + 16 + 2 * 64 * 4;
+         tile_align16((size_t)2 * qcap * 2) + 3 * 16 * 4 * 4 + 16 + 2 * 64 * 4;
+}
+
+// Sum of the popcounts of a 16-byte chunk.
+__device__ __forceinline__ uint32_t popc_chunk(const uint32_t x, const uint32_t y, const uint32_t z,
+                                               const uint32_t w) {
+  return (uint32_t)(__popc(x) + __popc(y) + __popc(z) + __popc(w));
 }
 
 // 16 bytes of an adjacency row: 8 uint16 neighbour ids.

@@ -179,10 +179,12 @@ class Plan:
 def trace_levels(adj, units: int = 8, based_nodes=None, groups=1, added_centers=0,
                  symmetric=False):
     """Per-level shader-cycle timings of the first `units` source tiles of one
-    evaluation: array [units, 64, 2] of (cycles since start, mode word)."""
+    evaluation: array [units, 64, 4] of (cycles since start at the level's
+    end, mode word, cycles when wave 0 finished the level's reads, cycles
+    when the barrier after them released)."""
     adj = L.c32(adj)
     g = _graph(adj, based_nodes, groups, added_centers, None, symmetric)
-    out = np.zeros((units, 64, 2), dtype=np.uint64)
+    out = np.zeros((units, 64, 4), dtype=np.uint64)
     r = L.GGResult()
     L.check(L.lib().gg_trace_levels(ctypes.byref(g), units,
                                     out.ctypes.data_as(L.u64p), ctypes.byref(r)))

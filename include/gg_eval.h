@@ -186,10 +186,12 @@ int gg_eval_features(int nodes, int degree, const int32_t *adj, int maxd, int32_
                      int32_t *hist, gg_result *out);
 
 /* ---- Profiling aid: one evaluation that records, for the first `units`
- * source tiles, per BFS level k (k < 64) the shader-clock cycles since the
- * tile started and a mode word (bit 0: push level; bits 1..: queue length
- * after a push level, or the reached-pair count after a pull level):
- * trace[(unit*64 + k)*2 + {0,1}]. */
+ * source tiles, per BFS level k (k < 64): [0] the shader-clock cycles since
+ * the tile started, at the level's end; [1] a mode word (bit 0: push level;
+ * bits 1..: queue length after a push level, or the reached-pair count after
+ * a pull level); [2] the cycles when wave 0 had issued the level's reads;
+ * [3] the cycles when the barrier after the reads released:
+ * trace[(unit*64 + k)*4 + {0,1,2,3}]. */
 int gg_trace_levels(const gg_graph *g, int units, uint64_t *trace, gg_result *out);
 
 /* ---- Bank-aware row order (no counterpart in the reference, which gathers
