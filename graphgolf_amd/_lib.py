@@ -122,6 +122,8 @@ def lib():
         "gg_eval_features": ([ctypes.c_int, ctypes.c_int, i32p, ctypes.c_int, i32p, i32p,
                               ctypes.POINTER(GGResult)], ctypes.c_int),
         "gg_row_order": ([ctypes.c_int, ctypes.c_int, i32p, ctypes.c_int, i32p, u64p], ctypes.c_int),
+        "gg_bank_layout": ([ctypes.c_int, ctypes.c_int, i32p, ctypes.c_int, ctypes.c_int64,
+                            ctypes.c_uint32, ctypes.c_int, i32p, i32p, u64p], ctypes.c_int),
         "gg_autotune": ([ctypes.c_int] * 6 + [ctypes.POINTER(ctypes.c_int),
                                               ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
         "gg_autotune_batch": ([ctypes.c_int, ctypes.POINTER(GGGraph), ctypes.c_int, i32p,

@@ -33,7 +33,7 @@ HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-I", INC, "-I", HERE, "-Wall",
               "-march=x86-64-v3"]
 
 HIP_SOURCES = ["gg_eval.hip", "gg_wide.hip", "gg_mlp.hip", "gg_bsa.hip", "gg_incr.hip"]
-HOST_SOURCES = ["gg_host.cpp", "gg_sa_sym.cpp"]
+HOST_SOURCES = ["gg_host.cpp", "gg_sa_sym.cpp", "gg_layout.cpp"]
 
 
 def _run(cmd):

@@ -177,13 +177,14 @@ class Plan:
 
 
 def trace_levels(adj, units: int = 8, based_nodes=None, groups=1, added_centers=0,
-                 symmetric=False):
+                 symmetric=False, relabel=None):
     """Per-level shader-cycle timings of the first `units` source tiles of one
     evaluation: array [units, 64, 4] of (cycles since start at the level's
     end, mode word, cycles when wave 0 finished the level's reads, cycles
     when the barrier after them released)."""
     adj = L.c32(adj)
-    g = _graph(adj, based_nodes, groups, added_centers, None, symmetric)
+    rel = L.c32(relabel) if relabel is not None else None
+    g = _graph(adj, based_nodes, groups, added_centers, rel, symmetric)
     out = np.zeros((units, 64, 4), dtype=np.uint64)
     r = L.GGResult()
     L.check(L.lib().gg_trace_levels(ctypes.byref(g), units,
@@ -219,9 +220,11 @@ def autotune_batch(graphs: Sequence[dict], reps: int = 3):
     arr = (L.GGGraph * len(graphs))()
     for i, item in enumerate(graphs):
         adj = L.c32(item["adj"])
-        keep.append(adj)
+        rel = item.get("relabel")
+        rel = L.c32(rel) if rel is not None else None
+        keep.append((adj, rel))
         arr[i] = _graph(adj, item.get("based_nodes"), item.get("groups", 1),
-                        item.get("added_centers", 0), None, item.get("symmetric", False))
+                        item.get("added_centers", 0), rel, item.get("symmetric", False))
     cfg = np.zeros(4, dtype=np.int32)
     ms = ctypes.c_double(0)
     L.check(L.lib().gg_autotune_batch(len(graphs), arr, reps, cfg.ctypes.data_as(L.i32p),
@@ -278,6 +281,23 @@ def row_order(adj, word_bits: int = 16):
     L.check(L.lib().gg_row_order(n, d, adj.ctypes.data_as(L.i32p), word_bits,
                                  rows.ctypes.data_as(L.i32p), cost.ctypes.data_as(L.u64p)))
     return rows, (int(cost[0]), int(cost[1]))
+
+
+def bank_layout(adj, word_bits: int = 16, iterations: int = 0, seed: int = 1, threads: int = 0):
+    """Bank-balanced layout of one graph (include/gg_eval.h, gg_bank_layout):
+    (relabel, adj_ordered, (cost of the given table, cost of the layout)).
+    Evaluate ``adj_ordered`` with ``relabel``: the same graph, renumbered and
+    with each row's entries in slot order; results equal those of ``adj``."""
+    adj = L.c32(adj)
+    n, d = adj.shape
+    relabel = np.zeros(n, dtype=np.int32)
+    out = np.zeros((n, d), dtype=np.int32)
+    cost = np.zeros(2, dtype=np.uint64)
+    L.check(L.lib().gg_bank_layout(n, d, adj.ctypes.data_as(L.i32p), word_bits, int(iterations),
+                                   int(seed) & 0xffffffff, int(threads),
+                                   relabel.ctypes.data_as(L.i32p), out.ctypes.data_as(L.i32p),
+                                   cost.ctypes.data_as(L.u64p)))
+    return relabel, out, (int(cost[0]), int(cost[1]))
 
 
 def set_word_bits(bits: int) -> None:

@@ -212,6 +212,27 @@ int gg_set_row_order(int mode);
 int gg_row_order(int nodes, int degree, const int32_t *adj, int word_bits, int32_t *rows,
                  uint64_t cost[2]);
 
+/* ---- Bank-balanced layout (no counterpart in the reference).  Computes
+ * once for a graph (nodes <= 65536, degree <= 32) a numbering and an order
+ * of each row's entries under which the pull levels' gathers meet few LDS
+ * bank conflicts with `word_bits`-wide reach words.  Every numbering gives
+ * the same diameter and S.  Vertices trade ids only with vertices whose
+ * reach words share their bank, chosen by simulated annealing so that the
+ * 32 rows a half-wave gathers together hold about equally many entries on
+ * every bank; the entries of each 32-row group are then ordered slot by
+ * slot by bipartite matching of rows to banks.
+ *   relabel[v]  (out) new id of v (pass as gg_graph.relabel);
+ *   adj_out     (out) [nodes][degree]: adj with each row's entries in slot
+ *               order (the same graph);
+ *   cost        (out) conflict cost, as gg_row_order defines it, of the given
+ *               table and of the layout.
+ * iterations: annealing swaps (0: 80 per vertex); threads: host threads
+ * (0: all).  Evaluate the result with row order mode 0 (gg_set_row_order)
+ * to keep the order, or let the tuner choose. */
+int gg_bank_layout(int nodes, int degree, const int32_t *adj, int word_bits, int64_t iterations,
+                   uint32_t seed, int threads, int32_t *relabel, int32_t *adj_out,
+                   uint64_t cost[2]);
+
 #ifdef __cplusplus
 }
 #endif
