@@ -191,7 +191,9 @@ int gg_eval_features(int nodes, int degree, const int32_t *adj, int maxd, int32_
  * bits 1..: queue length after a push level, or the reached-pair count after
  * a pull level); [2] the cycles when wave 0 had issued the level's reads;
  * [3] the cycles when the barrier after the reads released:
- * trace[(unit*64 + k)*4 + {0,1,2,3}]. */
+ * trace[(unit*64 + k)*4 + {0,1,2,3}].  Record k = 0 holds where the unit ran
+ * ([1]: XCC_ID << 32 | HW_ID hardware registers) and the shader clock at
+ * its start ([2]) and at level 1's start ([3]). */
 int gg_trace_levels(const gg_graph *g, int units, uint64_t *trace, gg_result *out);
 
 /* ---- Bank-aware row order (no counterpart in the reference, which gathers
