@@ -375,7 +375,7 @@ def apply_layout(it, E):
     n, d = it["adj"].shape
     bits = E.tune_lookup(n, d, it["based_nodes"], it["groups"])["word_bits"]
     t = time.perf_counter()
-    relabel, adj, cost = E.bank_layout(it["adj"], bits)
+    relabel, adj, cost = E.bank_layout(it["adj"], bits, iterations=300 * n)
     secs = time.perf_counter() - t
     it["adj_given"] = it["adj"]
     it["adj"], it["relabel"] = adj, relabel

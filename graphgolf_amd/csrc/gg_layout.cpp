@@ -75,25 +75,28 @@ uint64_t table_cost(const std::vector<uint32_t> &rows, int n, int dp, int shift)
 }
 
 // Slot order of one group by per-slot bipartite matching (rows to banks).
+// In slot s, a bank with more entries left than the slots after this one
+// must take the excess now (its lower bound lb); every other bank takes at
+// most one row.  The lower bounds are met first by augmenting paths from
+// the banks' side, then the remaining rows are matched from the rows' side
+// (banks with the most remaining entries tried first); augmenting never
+// lowers a bank's load, so the bounds stay met.  A row left without a bank
+// takes its entry on the slot's least-loaded bank.
 struct GroupOrder {
   int d, shift;
   int rows_in;
   uint32_t rem[32][32];  // remaining entries of each row (new ids)
   int nrem[32];
+  int cnt[32][32];  // cnt[r][b]: remaining entries of row r on bank b
   int load[32], cap[32], who[32][32], nwho[32], order_b[32], assigned[32];
-  int vis[32], stamp = 0;
+  int vis_b[32], vis_r[32], stamp = 0;
 
-  bool has_bank(int r, int b) const {
-    for (int i = 0; i < nrem[r]; ++i)
-      if ((int)(lds_word(rem[r][i], shift) & 31u) == b) return true;
-    return false;
-  }
-  // Kuhn's augmenting path from row r; banks tried most-remaining first.
+  // Row-side augmenting path from row r (Kuhn), bank loads bounded by cap.
   bool try_row(int r) {
     for (int oi = 0; oi < 32; ++oi) {
       const int b = order_b[oi];
-      if (vis[b] == stamp || !has_bank(r, b)) continue;
-      vis[b] = stamp;
+      if (vis_b[b] == stamp || !cnt[r][b]) continue;
+      vis_b[b] = stamp;
       if (load[b] < cap[b]) {
         who[b][nwho[b]++] = r;
         ++load[b];
@@ -111,49 +114,85 @@ struct GroupOrder {
     }
     return false;
   }
+  // Bank-side augmenting path: one more row for bank b.  A free row is taken
+  // directly; an assigned row is taken if its bank can get another row.
+  bool try_bank(int b) {
+    for (int r = 0; r < rows_in; ++r) {
+      if (vis_r[r] == stamp || !cnt[r][b] || assigned[r] == b) continue;
+      vis_r[r] = stamp;
+      const int b2 = assigned[r];
+      if (b2 < 0 || try_bank(b2)) {
+        if (b2 >= 0) {  // r leaves b2 (whose load was restored by the recursion)
+          for (int k = 0; k < nwho[b2]; ++k)
+            if (who[b2][k] == r) {
+              who[b2][k] = who[b2][--nwho[b2]];
+              break;
+            }
+          --load[b2];
+        }
+        who[b][nwho[b]++] = r;
+        ++load[b];
+        assigned[r] = b;
+        return true;
+      }
+    }
+    return false;
+  }
   // rows_new[r][0..d) in, reordered in place.
   void run(uint32_t *rows_new, int dp) {
+    std::memset(cnt, 0, sizeof cnt);
     for (int r = 0; r < rows_in; ++r) {
       nrem[r] = d;
-      for (int j = 0; j < d; ++j) rem[r][j] = rows_new[(size_t)r * dp + j];
+      for (int j = 0; j < d; ++j) {
+        rem[r][j] = rows_new[(size_t)r * dp + j];
+        ++cnt[r][lds_word(rem[r][j], shift) & 31u];
+      }
     }
-    std::memset(vis, 0, sizeof vis);
+    std::memset(vis_b, 0, sizeof vis_b);
+    std::memset(vis_r, 0, sizeof vis_r);
     for (int s = 0; s < d; ++s) {
       const int left = d - s - 1;  // slots after this one
-      int remb[32] = {0};
+      int remb[32] = {0}, lb[32];
       for (int r = 0; r < rows_in; ++r)
-        for (int i = 0; i < nrem[r]; ++i) ++remb[lds_word(rem[r][i], shift) & 31u];
+        for (int b = 0; b < 32; ++b) remb[b] += cnt[r][b];
       for (int b = 0; b < 32; ++b) {
         order_b[b] = b;
-        // A bank with more entries left than slots after this one must take
-        // the excess now.
-        cap[b] = std::max(1, remb[b] - left);
+        lb[b] = std::max(0, remb[b] - left);
         load[b] = 0;
         nwho[b] = 0;
       }
+      for (int r = 0; r < rows_in; ++r) assigned[r] = -1;
+      // Lower bounds first (most constrained banks first).
       std::stable_sort(order_b, order_b + 32, [&](int a, int b) { return remb[a] > remb[b]; });
+      for (int oi = 0; oi < 32; ++oi) {
+        const int b = order_b[oi];
+        for (int k = 0; k < lb[b]; ++k) {
+          ++stamp;
+          if (!try_bank(b)) break;
+        }
+      }
+      for (int b = 0; b < 32; ++b) cap[b] = std::max(1, lb[b]);
       for (int r = 0; r < rows_in; ++r) {
-        assigned[r] = -1;
+        if (assigned[r] >= 0) continue;
         ++stamp;
         try_row(r);
       }
       for (int r = 0; r < rows_in; ++r) {
-        int pick = -1;
-        if (assigned[r] >= 0) {
-          for (int i = 0; i < nrem[r] && pick < 0; ++i)
-            if ((int)(lds_word(rem[r][i], shift) & 31u) == assigned[r]) pick = i;
-        } else {  // no matching: the entry on this slot's least-loaded bank
+        int b = assigned[r];
+        if (b < 0) {  // no bank within bounds: the slot's least-loaded bank
           int best = 1 << 30;
-          for (int i = 0; i < nrem[r]; ++i) {
-            const int b = (int)(lds_word(rem[r][i], shift) & 31u);
-            if (load[b] < best) {
-              best = load[b];
-              pick = i;
+          for (int bb = 0; bb < 32; ++bb)
+            if (cnt[r][bb] && load[bb] < best) {
+              best = load[bb];
+              b = bb;
             }
-          }
-          ++load[lds_word(rem[r][pick], shift) & 31u];
+          ++load[b];
         }
+        int pick = -1;
+        for (int i = 0; i < nrem[r] && pick < 0; ++i)
+          if ((int)(lds_word(rem[r][i], shift) & 31u) == b) pick = i;
         rows_new[(size_t)r * dp + s] = rem[r][pick];
+        --cnt[r][b];
         rem[r][pick] = rem[r][--nrem[r]];
       }
     }
