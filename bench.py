@@ -5,6 +5,10 @@
 A step is one exact evaluation (diameter + distance sum S) of the workload
 graph(s), with the adjacency already resident in HBM: one replay of a captured
 hipGraph (adjacency transposition + tile BFS + per-graph combine + result copy).
+A single general graph is first numbered by the bank-balanced layout
+(gg_bank_layout, once, before timing; `config.layout` reports its time and
+bank cost): an isomorphic numbering with each row's entries in slot order,
+evaluated in full in every step (--layout 0 times the graph as generated).
 
 Prints ONE JSON line.  `value` is evaluations per second.  The roofline
 block prices the tile-BFS kernel on its binding memory, LDS: the rocprofv3
@@ -321,11 +325,15 @@ def read_traffic(workload):
 
 def gather_ceiling_GBps(word_bits, ordered):
     """The LDS's measured random-gather throughput for these words
-    (profiles/lds_gather_floor.md): cycles per 64-lane read with the 2-way
-    conflicts the bank-aware row order leaves, or for random ids; capped at
-    the LDS width."""
+    (profiles/lds_gather_floor.md): cycles per 64-lane read with every
+    half-wave on 32 distinct banks (ordered == "layout": the bank-balanced
+    layout leaves about 1.2 words on a slot's busiest bank), with the 2-way
+    conflicts the device row order leaves (True), or for random ids; capped
+    at the LDS width."""
     if word_bits == 64:
         cycles = 3.5 if ordered else 6.7
+    elif ordered == "layout":
+        cycles = 2.890
     else:
         cycles = 4.237 if ordered else 6.627
     per_cu = min(128.0, 64 * (word_bits // 8) / cycles)  # B/clk
@@ -338,9 +346,11 @@ def lds_roofline(workload, kernel_ms, traffic, canonical_bytes, word_bits=16, or
     from the committed summary of this workload) over this run's kernel time
     (HIP events on the evaluator's stream), against 256 CUs x 128 B/clk x
     2.4 GHz.  Random gathers cannot use the LDS's full width: the measured
-    gather ceiling for these words (16-bit: one 64-lane ds_read_u16 per 4.24
-    cycles per CU with the 2-way bank conflicts the row order leaves; 64-bit
-    random ids: one ds_read_b64 per 6.7 cycles) is reported beside it.
+    gather ceiling for these words (16-bit: one 64-lane ds_read_u16 per 2.89
+    cycles per CU with every half-wave on distinct banks, as the bank-balanced
+    layout nearly achieves, or per 4.24 cycles with the 2-way bank conflicts
+    the device row order leaves; 64-bit random ids: one ds_read_b64 per 6.7
+    cycles) is reported beside it.
     `traffic` is the measured HBM bytes per launch; the canonical bytes of
     the reference's single-tile formulation are kept as `canonical_bytes`."""
     name = f"lds_{workload}.md"
@@ -385,7 +395,10 @@ def apply_layout(it, E):
 
 def _rows_ordered(cfg, items):
     """Whether the plan ran the bank-aware row order (mode 2, or mode 1 for
-    graphs of at least 1024 source tiles at this word width)."""
+    graphs of at least 1024 source tiles at this word width); "layout" for a
+    graph numbered by the bank-balanced layout."""
+    if any("adj_given" in it for it in items):
+        return "layout"
     if cfg["row_order"] == 2:
         return True
     if cfg["row_order"] == 0:
