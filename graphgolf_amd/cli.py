@@ -143,7 +143,7 @@ def run_general(argv, evaluator=None, out=sys.stdout, incremental=None):
     """The general program (general/main.c:317-497).  Returns the SaResult
     (edges = the best graph).  `evaluator` (adj, based_nodes, groups,
     added_centers -> (connected, diameter, S)) replaces the device evaluator
-    when given.  `incremental` (None: from n >= 4096 with degree <= 16 on the
+    when given.  `incremental` (None: from n >= 4096 with degree <= 32 on the
     device evaluator; groups = 1 without centres) scores proposals with the
     incremental scorer; its results are the same exact integers, so the run
     and the output file are the same."""
@@ -232,7 +232,7 @@ def run_general(argv, evaluator=None, out=sys.stdout, incremental=None):
                                    detect_temp=detect)
     else:
         if incremental is None:
-            incremental = evaluator is None and nodes >= 4096 and degree <= 16
+            incremental = evaluator is None and nodes >= 4096 and degree <= 32
         one = None if evaluator is None else (lambda adj: evaluator(adj, nodes, 1, 0))
         r = SA.general_program(edges, nodes, degree, seed, ncalcs, halfway=halfway,
                                max_temp=max_t, min_temp=min_t, hill_climbing=hill,

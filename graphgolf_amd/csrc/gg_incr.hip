@@ -3,12 +3,13 @@
 //
 // Per move, on the evaluator's stream, with no host round trip before the
 // result copy:
-//   1. four probe units (gg_tile_bfs_kernel, kTileProbe) on the current
-//      graph, one per endpoint p with sources {p} + (N(p) minus its removed
-//      partner): per vertex s, the level L0 at which p reaches s and the first
-//      level L1 at which one of p's other neighbours does.  Since
-//      |d(s,w) - d(s,p)| <= 1 for a neighbour w, "p has a parent other than
-//      its partner at the level before it, seen from s" is L1 < L0.
+//   1. probe units (gg_tile_bfs_kernel, kTileProbe) on the current graph,
+//      per endpoint p ceil((d-1)/15) units, each with sources p and up to 15
+//      of p's neighbours other than its removed partner: per vertex s, the
+//      level L0 at which p reaches s and, per unit, the first level L1 at
+//      which one of its other sources does.  Since |d(s,w) - d(s,p)| <= 1 for
+//      a neighbour w, "p has a parent other than its partner at the level
+//      before it, seen from s" is (the least L1 of p's units) < L0.
 //   2. the adjacency patch (four entries) that makes the moved graph;
 //   3. gg_incr_flag_kernel: the sources whose distances can change (the
 //      test in include/gg_incr.h), appended to a list, with the sum of their
@@ -48,8 +49,11 @@ using gg::set_error;
 
 namespace {
 
-constexpr int kProbes = 4;
+constexpr int kProbes = 4;        // endpoints of a move
 constexpr int kWord = 16;         // sources per unit (uint16 reach words)
+constexpr int kMaxDegree = 32;    // rows of at most four 16-byte chunks
+// Probe units per endpoint: the endpoint plus 15 of its other neighbours each.
+constexpr int probe_units(int d) { return (d - 1 + kWord - 2) / (kWord - 1); }
 constexpr int kLocalHist = 64;    // eccentricities histogrammed in LDS by the flag kernel
 constexpr double kInlineSrc = 0.3;
 
@@ -60,9 +64,11 @@ struct IncrResult {
 };
 
 // What the flag kernel needs of a move: the probe index of each endpoint of
-// the two added edges (probes 0..3 are u1, v1, u2, v2).
+// the two added edges (probes 0..3 are u1, v1, u2, v2), and the probe units
+// per endpoint (endpoint p's units are p*pu .. p*pu + pu - 1).
 struct MoveProbes {
   int32_t add[4];
+  int32_t pu;
 };
 
 struct Patch {
@@ -104,8 +110,11 @@ gg_incr_flag_kernel(int n, const uint16_t *__restrict__ l0, const uint16_t *__re
     bool par[kProbes];
 #pragma unroll
     for (int p = 0; p < kProbes; ++p) {
-      L0[p] = l0[(size_t)p * n + s];
-      par[p] = l1[(size_t)p * n + s] < L0[p];
+      const size_t u0 = (size_t)(p * mv.pu);
+      L0[p] = l0[u0 * n + s];
+      int L1 = l1[u0 * n + s];
+      for (int k = 1; k < mv.pu; ++k) L1 = min(L1, (int)l1[(u0 + k) * n + s]);
+      par[p] = L1 < L0[p];
     }
     // Removed edges (u1,v1) = probes (0,1) and (u2,v2) = probes (2,3).
 #pragma unroll
@@ -293,9 +302,10 @@ struct gg_incr {
   uint16_t *adjR = nullptr;          // [n][dp] + 64 bytes (current graph; patched while pending)
   int32_t *flags = nullptr;          // [2]
   GraphDev *gdesc = nullptr;         // [0] probe, [1] list, [2] src (list), [3] src (all vertices)
-  UnitDesc *pdesc = nullptr;         // [4] probe units
-  uint16_t *probe_src = nullptr;     // [4][16]
-  uint16_t *l0 = nullptr, *l1 = nullptr;  // [4][n]
+  int pu = 1;                        // probe units per endpoint
+  UnitDesc *pdesc = nullptr;         // [4 * pu] probe units
+  uint16_t *probe_src = nullptr;     // [4 * pu][16]
+  uint16_t *l0 = nullptr, *l1 = nullptr;  // [4 * pu][n]
   uint16_t *aff = nullptr;           // [n]
   int32_t *aff_count = nullptr;      // [1] (+ [1] = n, the all-vertices count)
   unsigned long long *aff_sum_old = nullptr;
@@ -380,8 +390,8 @@ int run_src(gg_incr *s, int idx) {
 extern "C" {
 
 gg_incr *gg_incr_create(int nodes, int degree, const int32_t *adj) {
-  if (!adj || nodes < 3 || nodes > 65536 || degree < 2 || degree > kWord) {
-    set_error(GG_ERR_INVALID, "incremental scoring needs 3 <= nodes <= 65536, 2 <= degree <= 16");
+  if (!adj || nodes < 3 || nodes > 65536 || degree < 2 || degree > kMaxDegree) {
+    set_error(GG_ERR_INVALID, "incremental scoring needs 3 <= nodes <= 65536, 2 <= degree <= 32");
     return nullptr;
   }
   if (check_graph(nodes, degree, adj)) return nullptr;
@@ -391,8 +401,8 @@ gg_incr *gg_incr_create(int nodes, int degree, const int32_t *adj) {
 }  // extern "C"
 
 gg_incr *gg::incr_create_unchecked(int nodes, int degree, const int32_t *adj) {
-  if (!adj || nodes < 3 || nodes > 65536 || degree < 2 || degree > kWord) {
-    set_error(GG_ERR_INVALID, "incremental scoring needs 3 <= nodes <= 65536, 2 <= degree <= 16");
+  if (!adj || nodes < 3 || nodes > 65536 || degree < 2 || degree > kMaxDegree) {
+    set_error(GG_ERR_INVALID, "incremental scoring needs 3 <= nodes <= 65536, 2 <= degree <= 32");
     return nullptr;
   }
   if (gg_init(-1)) return nullptr;
@@ -406,6 +416,8 @@ gg_incr *gg::incr_create_unchecked(int nodes, int degree, const int32_t *adj) {
   s->n = n;
   s->d = d;
   s->dp = dp;
+  s->pu = probe_units(d);
+  const int nprobe = kProbes * s->pu;
   s->stream = gg::engine_stream();
   s->adj.assign(adj, adj + (size_t)n * d);
   if (gg::tile_shape(kWord, n, &s->block, &s->vpt, &s->n_cap, &s->qcap, &s->lds)) return fail();
@@ -432,18 +444,18 @@ gg_incr *gg::incr_create_unchecked(int nodes, int degree, const int32_t *adj) {
   const int units = (n + kWord - 1) / kWord;
   s->grid = std::min(s->grid, units);
   if (dev_alloc(&s->adjR, (size_t)n * dp + 32) || dev_alloc(&s->flags, 2) ||
-      dev_alloc(&s->gdesc, 4) || dev_alloc(&s->pdesc, kProbes) ||
-      dev_alloc(&s->probe_src, kProbes * kWord) || dev_alloc(&s->l0, (size_t)kProbes * n) ||
-      dev_alloc(&s->l1, (size_t)kProbes * n) || dev_alloc(&s->aff, (size_t)n) ||
+      dev_alloc(&s->gdesc, 4) || dev_alloc(&s->pdesc, nprobe) ||
+      dev_alloc(&s->probe_src, nprobe * kWord) || dev_alloc(&s->l0, (size_t)nprobe * n) ||
+      dev_alloc(&s->l1, (size_t)nprobe * n) || dev_alloc(&s->aff, (size_t)n) ||
       dev_alloc(&s->aff_count, 2) || dev_alloc(&s->aff_sum_old, 1) ||
       dev_alloc(&s->hist, (size_t)n + 2) || dev_alloc(&s->removed_hist, (size_t)n + 2) ||
       dev_alloc(&s->src_sum, (size_t)n) || dev_alloc(&s->new_sum, (size_t)n) ||
       dev_alloc(&s->src_ecc, (size_t)n) || dev_alloc(&s->new_ecc, (size_t)n) ||
-      dev_alloc(&s->outs, (size_t)std::max(units, kProbes)) || dev_alloc(&s->res, 1))
+      dev_alloc(&s->outs, (size_t)std::max(units, nprobe)) || dev_alloc(&s->res, 1))
     return fail();
   if (hipHostMalloc((void **)&s->res_h, sizeof(IncrResult)) != hipSuccess ||
-      hipHostMalloc((void **)&s->probe_src_h, kProbes * kWord * sizeof(uint16_t)) != hipSuccess ||
-      hipHostMalloc((void **)&s->pdesc_h, kProbes * sizeof(UnitDesc)) != hipSuccess) {
+      hipHostMalloc((void **)&s->probe_src_h, nprobe * kWord * sizeof(uint16_t)) != hipSuccess ||
+      hipHostMalloc((void **)&s->pdesc_h, nprobe * sizeof(UnitDesc)) != hipSuccess) {
     set_error(GG_ERR_HIP, "hipHostMalloc failed");
     return fail();
   }
@@ -568,6 +580,8 @@ int gg_incr_eval(gg_incr *s, const int32_t rem[4], const int32_t add[4], gg_resu
     return -1;
   };
   MoveProbes mv;
+  mv.pu = s->pu;
+  const int nprobe = kProbes * s->pu;
   int seen[4] = {0, 0, 0, 0};
   for (int i = 0; i < 4; ++i) {
     const int p = probe_of(add[i]);
@@ -602,24 +616,29 @@ int gg_incr_eval(gg_incr *s, const int32_t rem[4], const int32_t add[4], gg_resu
     redo.val[i] = nb;
     undo.val[i] = partner[i];
   }
-  // Probe sources: the endpoint, then its neighbours other than its partner.
+  // Probe sources: per unit the endpoint, then up to 15 of its neighbours
+  // other than its partner.
   for (int i = 0; i < 4; ++i) {
-    uint16_t *ps = s->probe_src_h + i * kWord;
-    int k = 0;
-    ps[k++] = (uint16_t)P[i];
     const int32_t *row = s->adj.data() + (size_t)P[i] * d;
-    for (int j = 0; j < d; ++j)
-      if (row[j] != partner[i]) ps[k++] = (uint16_t)row[j];
-    s->pdesc_h[i] = UnitDesc{0, i * kWord, k, 0};
+    int j = 0;
+    for (int u = 0; u < s->pu; ++u) {
+      const int unit = i * s->pu + u;
+      uint16_t *ps = s->probe_src_h + unit * kWord;
+      int k = 0;
+      ps[k++] = (uint16_t)P[i];
+      for (; j < d && k < kWord; ++j)
+        if (row[j] != partner[i]) ps[k++] = (uint16_t)row[j];
+      s->pdesc_h[unit] = UnitDesc{0, unit * kWord, k, 0};
+    }
   }
   hipStream_t st = s->stream;
-  GG_HIP(hipMemcpyAsync(s->probe_src, s->probe_src_h, kProbes * kWord * sizeof(uint16_t),
+  GG_HIP(hipMemcpyAsync(s->probe_src, s->probe_src_h, nprobe * kWord * sizeof(uint16_t),
                         hipMemcpyHostToDevice, st));
-  GG_HIP(hipMemcpyAsync(s->pdesc, s->pdesc_h, kProbes * sizeof(UnitDesc), hipMemcpyHostToDevice, st));
+  GG_HIP(hipMemcpyAsync(s->pdesc, s->pdesc_h, nprobe * sizeof(UnitDesc), hipMemcpyHostToDevice, st));
   const int bins = s->diameter + 2;
   hipLaunchKernelGGL(gg_incr_clear_kernel, dim3((bins + 255) / 256), dim3(256), 0, st,
                      s->aff_count, s->aff_sum_old, s->removed_hist, bins);
-  hipLaunchKernelGGL(s->K.probe, dim3(kProbes), dim3(s->block), s->lds, st,
+  hipLaunchKernelGGL(s->K.probe, dim3(nprobe), dim3(s->block), s->lds, st,
                      (const GraphDev *)s->gdesc, (const UnitDesc *)s->pdesc, s->outs, 0, s->n_cap,
                      s->qcap);
   hipLaunchKernelGGL(gg_incr_patch_kernel, dim3(1), dim3(64), 0, st, s->adjR, redo);

@@ -1,6 +1,6 @@
 """Incremental re-scoring of 2-opt moves (include/gg_incr.h).
 
-The scorer keeps one simple d-regular graph (d <= 16) on the device with
+The scorer keeps one simple d-regular graph (d <= 32) on the device with
 every vertex's distance sum and eccentricity, and scores a move -- two edges
 removed, two added between their endpoints -- by re-running the BFS only from
 the sources whose distances the move can change.  Every result is the exact

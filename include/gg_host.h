@@ -63,7 +63,7 @@ typedef struct gg_sa_params {
   int32_t log_capacity;  /* entries available in log_* arrays (0 = no log) */
   int32_t incremental;   /* general SA on the device evaluator: score each proposal
                             with the incremental scorer (include/gg_incr.h; simple graphs,
-                            degree <= 16).  Same exact results, so the same trajectory. */
+                            degree <= 32).  Same exact results, so the same trajectory. */
   int32_t detect_temp;   /* -D / -d: record the largest energy increase met
                             (stats->max_diff_energy); the grid SA then accepts every
                             proposal by the Metropolis test alone (grid/sa.c:231-233) */
@@ -256,7 +256,7 @@ typedef struct gg_bsa_params {
   int32_t grid_height;    /* > 0: a grid graph, vertex v at (v / height, v % height) */
   int32_t grid_length;    /* grid: max Manhattan length of every edge (grid/sa.c:196-201) */
   int32_t incremental;    /* score single-move candidates with the incremental scorer
-                             (include/gg_incr.h; simple graphs, degree <= 16) -- same
+                             (include/gg_incr.h; simple graphs, degree <= 32) -- same
                              exact results, faster at large n.  Without the surrogate:
                              every step; with it: the explore steps up to the first
                              ranked step (merged candidates flag nearly every source) */

@@ -19,12 +19,12 @@
  * distance, and level by level so does every vertex; the two removed edges
  * share no vertex, so their conditions combine.  Addition: an edge between
  * vertices at most one level apart shortens no path from s.)  The levels of
- * the four endpoints and of their neighbours come from four 16-source probe
- * BFS runs (each endpoint with its other neighbours), so the scorer needs
- * degree <= 16.
+ * the four endpoints and of their neighbours come from 16-source probe BFS
+ * runs: per endpoint ceil((degree-1)/15) of them, each with the endpoint and
+ * up to 15 of its other neighbours.
  *
  * Graphs: simple (no self-loop, no repeated edge), degree-regular adjacency
- * int32 [nodes][degree], 2 <= degree <= 16, nodes <= 65536, connected.
+ * int32 [nodes][degree], 2 <= degree <= 32, nodes <= 65536, connected.
  * Not thread-safe; one pending move at a time.
  */
 #ifndef GG_INCR_H

@@ -1,7 +1,8 @@
 """Incremental re-scoring (include/gg_incr.h) against full evaluation: for
 random 2-opt moves on random regular graphs, every scored move's (connected,
 diameter, S) equals gg_apsp() on the moved adjacency, through random
-accept/reject sequences, at n from 64 to 65536; disconnecting moves are
+accept/reject sequences, at n from 64 to 65536 and degrees 3 to 32 (one to
+three probe units per endpoint); disconnecting moves are
 reported disconnected; the scorer's own graph and state follow the accepted
 moves.  The oracle checks the small cases directly."""
 import numpy as np
@@ -80,13 +81,14 @@ def check_run(n, d, seed, moves, p_accept, use_oracle=False, oracle_every=0):
 
 
 @pytest.mark.parametrize("n,d,seed", [(64, 3, 1), (256, 8, 2), (500, 5, 3), (1024, 16, 4),
-                                      (2048, 12, 5)])
+                                      (2048, 12, 5), (512, 17, 11), (1024, 24, 9), (1024, 31, 13),
+                                      (2048, 32, 10)])
 def test_incremental_matches_full_small(n, d, seed):
     print(f"incr n={n} d={d} seed={seed}", flush=True)
     check_run(n, d, seed, moves=60, p_accept=0.5, use_oracle=n <= 1024)
 
 
-@pytest.mark.parametrize("n,d,seed", [(4096, 16, 6), (16384, 8, 7)])
+@pytest.mark.parametrize("n,d,seed", [(4096, 16, 6), (16384, 8, 7), (8192, 24, 12), (16384, 32, 14)])
 def test_incremental_matches_full_medium(n, d, seed):
     """Every move against the device's full evaluation, every 5th against
     the CPU oracle too."""
