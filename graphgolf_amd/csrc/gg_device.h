@@ -34,6 +34,8 @@ int tile_shape(int bits, int n, int *block, int *vpt, int *n_cap, int *qcap, siz
 // evaluator would not order (64-bit words, n below its threshold, rows of
 // more than four chunks, or ordering turned off).  Waits for completion.
 int order_rows_sync(uint16_t *adjR, int n, int dp, int bits, hipStream_t st);
+// Whether the row-order mode orders a table of n rows of dp entries.
+bool order_rows_applies(int n, int dp);
 // gg_incr_create without the graph checks, for callers that hold a simple
 // regular graph already (the batched SA validates its graph on entry).
 gg_incr *incr_create_unchecked(int nodes, int degree, const int32_t *adj);

@@ -1252,6 +1252,8 @@ void launch_order(const OrderDesc *descs, int count, int max_n, int nch_mask, hi
   }
 }
 
+bool order_rows_applies(int n, int dp) { return order_applies(n, kOrderMinNodes, dp); }
+
 int order_rows_sync(uint16_t *adjR, int n, int dp, int bits, hipStream_t st) {
   if (!order_applies(n, kOrderMinNodes, dp)) return GG_OK;
   const int shift = order_shift(bits);

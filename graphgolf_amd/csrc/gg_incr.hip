@@ -295,8 +295,13 @@ struct gg_incr {
   KernelSet K;
   hipStream_t stream = nullptr;
   std::vector<int32_t> adj;  // current graph, host copy [n][d]
-  // Device-row column of adjacency entry (v, j): the rows are put in the
-  // evaluator's bank-aware order once, and patches keep every position.
+  // Device id of vertex v: graphs the row-order mode orders are numbered by
+  // the bank-balanced layout (gg_bank_layout) once; the device state (rows,
+  // per-source values, lists) uses these ids throughout, the caller's ids
+  // are mapped at every move.
+  std::vector<int32_t> sig;  // [n]
+  // Device-row column of adjacency entry (v, j): the rows are put in a
+  // bank-aware order once, and patches keep every position.
   std::vector<uint16_t> slot;  // [n][d]
   // Device state.
   uint16_t *adjR = nullptr;          // [n][dp] + 64 bytes (current graph; patched while pending)
@@ -460,10 +465,24 @@ gg_incr *gg::incr_create_unchecked(int nodes, int degree, const int32_t *adj) {
     return fail();
   }
   // Row-major uint16 rows padded with the row's own id (a self-loop adds
-  // nothing to a reach set), as the evaluator's prep kernel writes them.
+  // nothing to a reach set), as the evaluator's prep kernel writes them:
+  // row sig[v] holds sig of v's neighbours, in the layout's slot order.
+  const bool layout = gg::order_rows_applies(n, dp);
+  std::vector<int32_t> ordered(adj, adj + (size_t)n * d);
+  s->sig.resize(n);
+  if (layout) {
+    uint64_t cost[2];
+    if (gg_bank_layout(n, d, adj, kWord, 0, 1, 0, s->sig.data(), ordered.data(), cost)) return fail();
+  } else {
+    std::iota(s->sig.begin(), s->sig.end(), 0);
+  }
+  const int32_t *sig = s->sig.data();
   std::vector<uint16_t> rows((size_t)n * dp + 32, 0);
-  for (int v = 0; v < n; ++v)
-    for (int j = 0; j < dp; ++j) rows[(size_t)v * dp + j] = (uint16_t)(j < d ? adj[(size_t)v * d + j] : v);
+  for (int v = 0; v < n; ++v) {
+    const int r = sig[v];
+    for (int j = 0; j < dp; ++j)
+      rows[(size_t)r * dp + j] = (uint16_t)(j < d ? sig[ordered[(size_t)v * d + j]] : r);
+  }
   std::vector<uint16_t> all(n);
   std::iota(all.begin(), all.end(), 0);
   GraphDev g[4];
@@ -501,16 +520,19 @@ gg_incr *gg::incr_create_unchecked(int nodes, int degree, const int32_t *adj) {
   std::vector<uint32_t> sums(n);
   auto upload = [&]() -> int {
     GG_HIP(hipMemcpy(s->adjR, rows.data(), rows.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
-    int orc = gg::order_rows_sync(s->adjR, n, dp, kWord, s->stream);
-    if (orc) return orc;
-    GG_HIP(hipMemcpy(rows.data(), s->adjR, (size_t)n * dp * sizeof(uint16_t), hipMemcpyDeviceToHost));
+    if (!layout) {
+      int orc = gg::order_rows_sync(s->adjR, n, dp, kWord, s->stream);
+      if (orc) return orc;
+      GG_HIP(hipMemcpy(rows.data(), s->adjR, (size_t)n * dp * sizeof(uint16_t), hipMemcpyDeviceToHost));
+    }
     s->slot.assign((size_t)n * d, 0);
     for (int v = 0; v < n; ++v) {
       uint32_t taken = 0;  // dp <= 32 whenever the rows were reordered
+      const size_t r = (size_t)sig[v];
       for (int j = 0; j < d; ++j) {
-        const uint16_t want = (uint16_t)adj[(size_t)v * d + j];
+        const uint16_t want = (uint16_t)sig[adj[(size_t)v * d + j]];
         int k = 0;
-        while (k < dp && ((k < 32 && ((taken >> k) & 1u)) || rows[(size_t)v * dp + k] != want)) ++k;
+        while (k < dp && ((k < 32 && ((taken >> k) & 1u)) || rows[r * dp + k] != want)) ++k;
         if (k == dp) return set_error(GG_ERR_STATE, "row %d lost neighbour %d", v, (int)want);
         if (k < 32) taken |= 1u << k;
         s->slot[(size_t)v * d + j] = (uint16_t)k;
@@ -612,9 +634,9 @@ int gg_incr_eval(gg_incr *s, const int32_t rem[4], const int32_t add[4], gg_resu
     s->pend_cols[i] = col;
     s->pend_vals[i] = nb;
     s->pend_old[i] = partner[i];
-    redo.at[i] = undo.at[i] = P[i] * dp + s->slot[(size_t)P[i] * d + col];
-    redo.val[i] = nb;
-    undo.val[i] = partner[i];
+    redo.at[i] = undo.at[i] = s->sig[P[i]] * dp + s->slot[(size_t)P[i] * d + col];
+    redo.val[i] = s->sig[nb];
+    undo.val[i] = s->sig[partner[i]];
   }
   // Probe sources: per unit the endpoint, then up to 15 of its neighbours
   // other than its partner.
@@ -625,9 +647,9 @@ int gg_incr_eval(gg_incr *s, const int32_t rem[4], const int32_t add[4], gg_resu
       const int unit = i * s->pu + u;
       uint16_t *ps = s->probe_src_h + unit * kWord;
       int k = 0;
-      ps[k++] = (uint16_t)P[i];
+      ps[k++] = (uint16_t)s->sig[P[i]];
       for (; j < d && k < kWord; ++j)
-        if (row[j] != partner[i]) ps[k++] = (uint16_t)row[j];
+        if (row[j] != partner[i]) ps[k++] = (uint16_t)s->sig[row[j]];
       s->pdesc_h[unit] = UnitDesc{0, unit * kWord, k, 0};
     }
   }
