@@ -957,11 +957,31 @@ extern "C" int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa
     for (int v = 0; v < nodes; ++v)
       if (rhash[v] < 0) return set_error(GG_ERR_INVALID, "grid %dx%d has no %d-fold base quadrant", w, h, groups);
   }
+  // A general graph the row order would order is numbered by the
+  // bank-balanced layout instead (gg_bank_layout), once: the plan relabels
+  // every candidate with it, and the driver's adjacency takes the layout's
+  // order of each row's entries (the same graph), so candidates' patches
+  // keep every row in that order.  The plan then orders nothing itself.
+  std::vector<int32_t> lay;
+  const int dp = (degree + 7) / 8 * 8;
+  if (groups == 1 && rhash.empty() && gg::order_rows_applies(nodes, dp)) {
+    int32_t cfg[4];
+    if ((rc = gg_tune_lookup(nodes, degree, nodes, 1, 0, E > 1, cfg))) return rc;
+    std::vector<int32_t> ordered(g.adj.size());
+    uint64_t cost[2];
+    lay.resize(nodes);
+    if ((rc = gg_bank_layout(nodes, degree, g.adj.data(), cfg[0], 0, 1, 0, lay.data(), ordered.data(),
+                             cost)))
+      return rc;
+    g.adj = ordered;
+  }
   gg_graph shape = {nodes, degree, nodes / groups, groups, 0, GG_GRAPH_SYMMETRIC, g.adj.data(),
-                    rhash.empty() ? nullptr : rhash.data()};
+                    !lay.empty() ? lay.data() : rhash.empty() ? nullptr : rhash.data()};
   const int max_patch = 4 * groups * P->combine;
   PlanHolder plan;
+  const int prev_order = lay.empty() ? -1 : gg_set_row_order(0);
   plan.p = gg_plan_create_patched(E, &shape, max_patch);
+  if (prev_order >= 0) gg_set_row_order(prev_order);
   if (!plan.p) return gg_last_error_code() ? gg_last_error_code() : GG_ERR_HIP;
   std::vector<gg_result> res(E);
   std::vector<int32_t> patches((size_t)E * max_patch * 2, -1);
