@@ -970,8 +970,9 @@ extern "C" int gg_sa_batched(int nodes, int degree, int32_t *edges, const gg_bsa
     std::vector<int32_t> ordered(g.adj.size());
     uint64_t cost[2];
     lay.resize(nodes);
-    if ((rc = gg_bank_layout(nodes, degree, g.adj.data(), cfg[0], 0, 1, 0, lay.data(), ordered.data(),
-                             cost)))
+    // A light annealing (20 swaps per vertex): the run pays for it once.
+    if ((rc = gg_bank_layout(nodes, degree, g.adj.data(), cfg[0], (int64_t)nodes * 20, 1, 0, lay.data(),
+                             ordered.data(), cost)))
       return rc;
     g.adj = ordered;
   }

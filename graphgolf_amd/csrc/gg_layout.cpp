@@ -50,27 +50,32 @@ struct Rng {
 };
 
 // Cost of a row table rows[n][dp] (new ids): sum over groups and slots of
-// the most distinct LDS words on one bank.
-uint64_t table_cost(const std::vector<uint32_t> &rows, int n, int dp, int shift) {
-  uint64_t total = 0;
-  for (int g0 = 0; g0 < n; g0 += 32) {
-    const int rows_in = std::min(32, n - g0);
-    for (int s = 0; s < dp; ++s) {
-      uint32_t words[32];
-      int cnt[32] = {0};
-      int m = 0;
-      for (int r = 0; r < rows_in; ++r) {
-        const uint32_t w = lds_word(rows[(size_t)(g0 + r) * dp + s], shift);
-        bool seen = false;
-        for (int k = 0; k < m; ++k) seen |= words[k] == w;
-        if (!seen) {
-          words[m++] = w;
-          ++cnt[w & 31];
-        }
+// the most distinct LDS words on one bank, over `nt` threads.
+uint64_t table_cost(const std::vector<uint32_t> &rows, int n, int dp, int shift, int nt) {
+  const int ngroups = (n + 31) / 32;
+  std::vector<uint64_t> part(nt, 0);
+  auto work = [&](int t) {
+    uint64_t total = 0;
+    for (int g = t; g < ngroups; g += nt) {
+      const int g0 = g * 32, rows_in = std::min(32, n - g0);
+      for (int s = 0; s < dp; ++s) {
+        uint32_t words[32];
+        for (int r = 0; r < rows_in; ++r) words[r] = lds_word(rows[(size_t)(g0 + r) * dp + s], shift);
+        std::sort(words, words + rows_in);
+        int cnt[32] = {0};
+        for (int r = 0; r < rows_in; ++r)
+          if (r == 0 || words[r] != words[r - 1]) ++cnt[words[r] & 31];
+        total += (uint64_t)*std::max_element(cnt, cnt + 32);
       }
-      total += (uint64_t)*std::max_element(cnt, cnt + 32);
     }
-  }
+    part[t] = total;
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nt; ++t) pool.emplace_back(work, t);
+  work(0);
+  for (auto &th : pool) th.join();
+  uint64_t total = 0;
+  for (uint64_t x : part) total += x;
   return total;
 }
 
@@ -87,26 +92,34 @@ struct GroupOrder {
   int rows_in;
   uint32_t rem[32][32];  // remaining entries of each row (new ids)
   int nrem[32];
-  int cnt[32][32];  // cnt[r][b]: remaining entries of row r on bank b
-  int load[32], cap[32], who[32][32], nwho[32], order_b[32], assigned[32];
-  int vis_b[32], vis_r[32], stamp = 0;
+  int cnt[32][32];      // cnt[r][b]: remaining entries of row r on bank b
+  uint32_t bmask[32];   // rows with a remaining entry on bank b
+  int remb[32];         // remaining entries per bank
+  int load[32], cap[32], who[32][32], nwho[32], assigned[32];
+  int inv[32];          // banks by priority (most remaining entries first)
+  uint32_t rmask[32];   // per row: its banks as priority bits
+  uint32_t vis_b, vis_r;
 
-  // Row-side augmenting path from row r (Kuhn), bank loads bounded by cap.
+  // Row-side augmenting path from row r (Kuhn), bank loads bounded by cap;
+  // banks tried in priority order.
   bool try_row(int r) {
-    for (int oi = 0; oi < 32; ++oi) {
-      const int b = order_b[oi];
-      if (vis_b[b] == stamp || !cnt[r][b]) continue;
-      vis_b[b] = stamp;
+    uint32_t m = rmask[r] & ~vis_b;
+    while (m) {
+      const int k = __builtin_ctz(m);
+      m &= m - 1;
+      if (vis_b >> k & 1u) continue;
+      vis_b |= 1u << k;
+      const int b = inv[k];
       if (load[b] < cap[b]) {
         who[b][nwho[b]++] = r;
         ++load[b];
         assigned[r] = b;
         return true;
       }
-      for (int k = 0; k < nwho[b]; ++k) {
-        const int r2 = who[b][k];
+      for (int i = 0; i < nwho[b]; ++i) {
+        const int r2 = who[b][i];
         if (try_row(r2)) {
-          who[b][k] = r;
+          who[b][i] = r;
           assigned[r] = b;
           return true;
         }
@@ -117,15 +130,18 @@ struct GroupOrder {
   // Bank-side augmenting path: one more row for bank b.  A free row is taken
   // directly; an assigned row is taken if its bank can get another row.
   bool try_bank(int b) {
-    for (int r = 0; r < rows_in; ++r) {
-      if (vis_r[r] == stamp || !cnt[r][b] || assigned[r] == b) continue;
-      vis_r[r] = stamp;
+    uint32_t m = bmask[b] & ~vis_r;
+    while (m) {
+      const int r = __builtin_ctz(m);
+      m &= m - 1;
+      if ((vis_r >> r & 1u) || assigned[r] == b) continue;
+      vis_r |= 1u << r;
       const int b2 = assigned[r];
       if (b2 < 0 || try_bank(b2)) {
         if (b2 >= 0) {  // r leaves b2 (whose load was restored by the recursion)
-          for (int k = 0; k < nwho[b2]; ++k)
-            if (who[b2][k] == r) {
-              who[b2][k] = who[b2][--nwho[b2]];
+          for (int i = 0; i < nwho[b2]; ++i)
+            if (who[b2][i] == r) {
+              who[b2][i] = who[b2][--nwho[b2]];
               break;
             }
           --load[b2];
@@ -141,40 +157,48 @@ struct GroupOrder {
   // rows_new[r][0..d) in, reordered in place.
   void run(uint32_t *rows_new, int dp) {
     std::memset(cnt, 0, sizeof cnt);
+    std::memset(bmask, 0, sizeof bmask);
+    std::memset(remb, 0, sizeof remb);
     for (int r = 0; r < rows_in; ++r) {
       nrem[r] = d;
       for (int j = 0; j < d; ++j) {
         rem[r][j] = rows_new[(size_t)r * dp + j];
-        ++cnt[r][lds_word(rem[r][j], shift) & 31u];
+        const int b = (int)(lds_word(rem[r][j], shift) & 31u);
+        ++cnt[r][b];
+        ++remb[b];
+        bmask[b] |= 1u << r;
       }
     }
-    std::memset(vis_b, 0, sizeof vis_b);
-    std::memset(vis_r, 0, sizeof vis_r);
     for (int s = 0; s < d; ++s) {
       const int left = d - s - 1;  // slots after this one
-      int remb[32] = {0}, lb[32];
-      for (int r = 0; r < rows_in; ++r)
-        for (int b = 0; b < 32; ++b) remb[b] += cnt[r][b];
+      int lb[32], rank[32];
       for (int b = 0; b < 32; ++b) {
-        order_b[b] = b;
+        inv[b] = b;
         lb[b] = std::max(0, remb[b] - left);
         load[b] = 0;
         nwho[b] = 0;
       }
-      for (int r = 0; r < rows_in; ++r) assigned[r] = -1;
+      std::stable_sort(inv, inv + 32, [&](int a, int b) { return remb[a] > remb[b]; });
+      for (int k = 0; k < 32; ++k) rank[inv[k]] = k;
+      for (int r = 0; r < rows_in; ++r) {
+        assigned[r] = -1;
+        uint32_t m = 0;
+        for (int b = 0; b < 32; ++b)
+          if (cnt[r][b]) m |= 1u << rank[b];
+        rmask[r] = m;
+      }
       // Lower bounds first (most constrained banks first).
-      std::stable_sort(order_b, order_b + 32, [&](int a, int b) { return remb[a] > remb[b]; });
-      for (int oi = 0; oi < 32; ++oi) {
-        const int b = order_b[oi];
-        for (int k = 0; k < lb[b]; ++k) {
-          ++stamp;
+      for (int k = 0; k < 32; ++k) {
+        const int b = inv[k];
+        for (int i = 0; i < lb[b]; ++i) {
+          vis_r = 0;
           if (!try_bank(b)) break;
         }
       }
       for (int b = 0; b < 32; ++b) cap[b] = std::max(1, lb[b]);
       for (int r = 0; r < rows_in; ++r) {
         if (assigned[r] >= 0) continue;
-        ++stamp;
+        vis_b = 0;
         try_row(r);
       }
       for (int r = 0; r < rows_in; ++r) {
@@ -192,7 +216,8 @@ struct GroupOrder {
         for (int i = 0; i < nrem[r] && pick < 0; ++i)
           if ((int)(lds_word(rem[r][i], shift) & 31u) == b) pick = i;
         rows_new[(size_t)r * dp + s] = rem[r][pick];
-        --cnt[r][b];
+        if (--cnt[r][b] == 0) bmask[b] &= ~(1u << r);
+        --remb[b];
         rem[r][pick] = rem[r][--nrem[r]];
       }
     }
@@ -218,11 +243,14 @@ extern "C" int gg_bank_layout(int nodes, int degree, const int32_t *adj, int wor
   const int ngroups = (n + 31) / 32;
   auto key = [&](uint32_t v) { return (int)(lds_word(v, shift) & 31u); };
 
+  int nt = threads > 0 ? threads : (int)std::thread::hardware_concurrency();
+  nt = std::max(1, std::min(nt, std::max(1, ngroups / 16)));
+
   // Given table (identity numbering, padding = own id): its cost.
   std::vector<uint32_t> rows((size_t)n * dp);
   for (int v = 0; v < n; ++v)
     for (int j = 0; j < dp; ++j) rows[(size_t)v * dp + j] = j < d ? (uint32_t)adj[(size_t)v * d + j] : (uint32_t)v;
-  cost[0] = table_cost(rows, n, dp, shift);
+  cost[0] = table_cost(rows, n, dp, shift, nt);
 
   // 1. Relabel.  slot_of[v] = new id of v; vert_at[i] = vertex at new id i.
   std::vector<int32_t> slot_of(n), vert_at(n);
@@ -234,8 +262,6 @@ extern "C" int gg_bank_layout(int nodes, int degree, const int32_t *adj, int wor
   for (int i = 0; i < n; ++i)
     for (int b = 0; b < 32; ++b) H[(size_t)(i >> 5) * 32 + b] += hv[(size_t)i * 32 + b];
   if (iterations <= 0) iterations = (int64_t)n * 80;
-  int nt = threads > 0 ? threads : (int)std::thread::hardware_concurrency();
-  nt = std::max(1, std::min(nt, std::max(1, ngroups / 16)));
   const int epochs = 40;
   const int64_t per_thread = std::max<int64_t>(1, iterations / ((int64_t)epochs * nt));
   const double t0 = 4.0, t1 = 0.1;
@@ -314,7 +340,7 @@ extern "C" int gg_bank_layout(int nodes, int degree, const int32_t *adj, int wor
     work(0);
     for (auto &th : pool) th.join();
   }
-  cost[1] = table_cost(rows, n, dp, shift);
+  cost[1] = table_cost(rows, n, dp, shift, nt);
 
   // Outputs in the caller's numbering: row v lists its neighbours in the
   // slot order of its new row slot_of[v].
