@@ -4,8 +4,9 @@
 // Network: dims[0] -> dims[1] -> ... -> dims[L] with ReLU between layers and
 // a linear output (the predicted change of the distance sum for a swap).
 // Master parameters are FP32; every GEMM reads BF16 copies of its operands
-// and accumulates in FP32 with v_mfma_f32_32x32x8_bf16 (one 32x32 output tile
-// per wave).  Training minimises mean squared error with Adam.
+// and accumulates in FP32 with gfx950's v_mfma_f32_32x32x16_bf16 (64x64
+// output tiles per workgroup, operands staged in LDS).  Training minimises
+// mean squared error with Adam.
 //
 // No reference counterpart: the reference proposes swaps uniformly at random
 // (general/sa.c:217-369); this network only re-orders proposals, and every
@@ -25,7 +26,6 @@
 
 using gg::set_error;
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 // FP32 -> BF16 with round-to-nearest-even (NaN kept quiet).
@@ -42,40 +42,97 @@ __global__ void gg_mlp_to_bf16(const float *__restrict__ in, uint16_t *__restric
   if (i < n) out[i] = f2bf(in[i]);
 }
 
+// Rows of `cols` fp32 -> rows of `cols_p` BF16, the padding columns zero.
+__global__ void gg_mlp_to_bf16_rows(const float *__restrict__ in, int cols, uint16_t *__restrict__ out,
+                                    int cols_p, int rows) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows * cols_p) return;
+  const int r = i / cols_p, c = i - r * cols_p;
+  out[i] = c < cols ? f2bf(in[(size_t)r * cols + c]) : (uint16_t)0;
+}
+
 // C[M x N] (fp32, row-major, ldc) = op(A)[M x K] * op(B)[K x N] (+ bias[N])
 // (then ReLU when relu != 0); optionally also writes a BF16 copy (ldh).
 // op(A)(i,k) = transA ? A[k*lda + i] : A[i*lda + k]
 // op(B)(k,j) = transB ? B[j*ldb + k] : B[k*ldb + j]
-// M, N multiples of 32; K a multiple of 8.  One wave per 32x32 tile.
+// M, K, lda, ldb multiples of 8 (16-byte rows of BF16), N of 32.
 //
-// Operand layout of v_mfma_f32_32x32x8_bf16: lane l holds A(l%32, 4*(l/32)+e)
-// and B(4*(l/32)+e, l%32) for e = 0..3; accumulator register r of lane l is
-// D(8*(r/4) + 4*(l/32) + r%4, l%32).
-__global__ void __launch_bounds__(64)
+// One 256-thread workgroup per 64x64 tile of C, one 32x32 quarter per wave,
+// with gfx950's v_mfma_f32_32x32x16_bf16 (K = 16 per instruction).  Each
+// K step of 32 stages op(A)'s 64x32 block as As[i][k] and op(B)'s 32x64
+// block as Bs[j][k] in LDS (rows padded by 8 elements against bank
+// conflicts), read from global memory as 16-byte vectors in each operand's
+// own memory order (transposed operands are scattered into LDS), so that
+// every lane reads its 8 consecutive k of one row or column with one
+// 16-byte LDS read.  Operand layout of the MFMA: lane l holds A(l%32,
+// 8*(l/32)+e) and B(8*(l/32)+e, l%32), e = 0..7; accumulator register r of
+// lane l is D(8*(r/4) + 4*(l/32) + r%4, l%32).
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+constexpr int kGemmTile = 64, kGemmK = 32, kGemmLd = kGemmK + 8;
+
+__global__ void __launch_bounds__(256)
 gg_mlp_gemm(const uint16_t *__restrict__ A, int lda, int transA, const uint16_t *__restrict__ B,
             int ldb, int transB, float *__restrict__ C, int ldc, uint16_t *__restrict__ Cb, int ldh,
             const float *__restrict__ bias, int relu, int M, int N, int K) {
-  const int l = threadIdx.x;
-  const int m0 = blockIdx.x * 32, n0 = blockIdx.y * 32;
-  const int row = m0 + (l & 31), col = n0 + (l & 31);
-  const int kq = (l >> 5) * 4;
+  __shared__ __attribute__((aligned(16))) uint16_t As[kGemmTile][kGemmLd];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[kGemmTile][kGemmLd];
+  const int tid = threadIdx.x, l = tid & 63, w = tid >> 6;
+  const int m0 = blockIdx.x * kGemmTile, n0 = blockIdx.y * kGemmTile;
+  const int wm = (w >> 1) * 32, wn = (w & 1) * 32;  // this wave's quarter of the tile
   f32x16 acc;
 #pragma unroll
   for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
-  for (int k0 = 0; k0 < K; k0 += 8) {
-    bf16x4 a, b;
+  const u32x4 zero = {0u, 0u, 0u, 0u};
+  for (int k0 = 0; k0 < K; k0 += kGemmK) {
+    // Stage op(A) rows m0.., k0.. : 256 loads of 8 BF16.
+    if (!transA) {
+      const int i = tid >> 2, kk = (tid & 3) * 8;  // row i, k kk..kk+7
+      const int gi = m0 + i, gk = k0 + kk;
+      u32x4 v = zero;
+      if (gi < M && gk < K) v = *(const u32x4 *)(A + (size_t)gi * lda + gk);
+      *(u32x4 *)&As[i][kk] = v;
+    } else {
+      const int kk = tid >> 3, i = (tid & 7) * 8;  // k kk, rows i..i+7 (contiguous in memory)
+      const int gk = k0 + kk, gi = m0 + i;
+      u32x4 v = zero;
+      if (gk < K && gi < M) v = *(const u32x4 *)(A + (size_t)gk * lda + gi);
+      const uint16_t *e = (const uint16_t *)&v;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int k = k0 + kq + e;
-      a[e] = (short)(transA ? A[(size_t)k * lda + row] : A[(size_t)row * lda + k]);
-      b[e] = (short)(transB ? B[(size_t)col * ldb + k] : B[(size_t)k * ldb + col]);
+      for (int q = 0; q < 8; ++q) As[i + q][kk] = e[q];
     }
-    acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+    // Stage op(B) columns n0.., k0.. as Bs[j][k].
+    if (transB) {
+      const int j = tid >> 2, kk = (tid & 3) * 8;  // column j, k kk..kk+7 (contiguous)
+      const int gj = n0 + j, gk = k0 + kk;
+      u32x4 v = zero;
+      if (gj < N && gk < K) v = *(const u32x4 *)(B + (size_t)gj * ldb + gk);
+      *(u32x4 *)&Bs[j][kk] = v;
+    } else {
+      const int kk = tid >> 3, j = (tid & 7) * 8;  // k kk, columns j..j+7 (contiguous)
+      const int gk = k0 + kk, gj = n0 + j;
+      u32x4 v = zero;
+      if (gk < K && gj < N) v = *(const u32x4 *)(B + (size_t)gk * ldb + gj);
+      const uint16_t *e = (const uint16_t *)&v;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) Bs[j + q][kk] = e[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < kGemmK; ks += 16) {
+      const int kq = ks + 8 * (l >> 5);
+      const bf16x8 a = *(const bf16x8 *)&As[wm + (l & 31)][kq];
+      const bf16x8 b = *(const bf16x8 *)&Bs[wn + (l & 31)][kq];
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    }
+    __syncthreads();
   }
+  if (m0 + wm >= M || n0 + wn >= N) return;  // a quarter past the matrix
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
-    const int i = m0 + 8 * (r >> 2) + 4 * (l >> 5) + (r & 3);
-    const int j = n0 + (l & 31);
+    const int i = m0 + wm + 8 * (r >> 2) + 4 * (l >> 5) + (r & 3);
+    const int j = n0 + wn + (l & 31);
+    if (i >= M) continue;
     float v = acc[r];
     if (bias) v += bias[j];
     if (relu) v = fmaxf(v, 0.0f);
@@ -119,13 +176,25 @@ __global__ void gg_mlp_relu_back(const float *__restrict__ dH, const float *__re
   }
 }
 
-// db[j] = sum_i dZ[i][j] (one thread per column; M is small).
-__global__ void gg_mlp_colsum(const float *__restrict__ dZ, int M, int N, float *__restrict__ db) {
-  int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= N) return;
+// db[j] = sum_i dZ[i][j]: a block per 64 columns, 16 row groups of one
+// wave each (a wave reads 64 consecutive columns of a row), the groups'
+// partial sums added in a fixed order (the same result every run).
+__global__ void __launch_bounds__(1024)
+gg_mlp_colsum(const float *__restrict__ dZ, int M, int N, float *__restrict__ db) {
+  __shared__ float part[16][64];
+  const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int j = blockIdx.x * 64 + c;
   float s = 0.0f;
-  for (int i = 0; i < M; ++i) s += dZ[(size_t)i * N + j];
-  db[j] = s;
+  if (j < N)
+    for (int i = g; i < M; i += 16) s += dZ[(size_t)i * N + j];
+  part[g][c] = s;
+  __syncthreads();
+  if (g == 0 && j < N) {
+    float t = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) t += part[q][c];
+    db[j] = t;
+  }
 }
 
 __global__ void gg_mlp_adam(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m1,
@@ -162,7 +231,7 @@ struct Layer {
 
 struct gg_mlp {
   std::vector<Layer> layers;
-  int in_dim = 0, n_params = 0, cap_m = 0;
+  int in_dim = 0, in_p = 0, n_params = 0, cap_m = 0;  // in_p: in_dim rounded up to 8
   int step = 0;
   hipStream_t stream = nullptr;
   // flat parameter buffers (fp32 master, grads, Adam moments, bf16 weights)
@@ -201,7 +270,7 @@ static int mlp_reserve(gg_mlp *M, int m) {
   hipFree(M->t_f);
   hipFree(M->y_f);
   M->cap_m = 0;
-  MLP_HIP(hipMalloc(&M->x_b, (size_t)Mp * M->in_dim * 2));
+  MLP_HIP(hipMalloc(&M->x_b, (size_t)Mp * M->in_p * 2));
   MLP_HIP(hipMalloc(&M->x_f, (size_t)Mp * M->in_dim * 4));
   MLP_HIP(hipMalloc(&M->t_f, (size_t)Mp * 4));
   MLP_HIP(hipMalloc(&M->y_f, (size_t)Mp * 4));
@@ -223,23 +292,24 @@ static int mlp_reserve(gg_mlp *M, int m) {
 static void launch_gemm(hipStream_t s, const uint16_t *A, int lda, int tA, const uint16_t *B, int ldb,
                         int tB, float *C, int ldc, uint16_t *Cb, int ldh, const float *bias, int relu,
                         int Mr, int Nr, int K) {
-  hipLaunchKernelGGL(gg_mlp_gemm, dim3(Mr / 32, Nr / 32), dim3(64), 0, s, A, lda, tA, B, ldb, tB, C,
-                     ldc, Cb, ldh, bias, relu, Mr, Nr, K);
+  hipLaunchKernelGGL(gg_mlp_gemm, dim3((Mr + kGemmTile - 1) / kGemmTile, (Nr + kGemmTile - 1) / kGemmTile),
+                     dim3(256), 0, s, A, lda, tA, B, ldb, tB, C, ldc, Cb, ldh, bias, relu, Mr, Nr, K);
 }
 
 // Uploads x (m rows, fp32) and runs the forward pass; output in Z.back() col 0.
 static int mlp_forward(gg_mlp *M, const float *x_host, int m, bool device_x) {
   int rc = mlp_reserve(M, m);
   if (rc) return rc;
-  const int Mp = pad32(m), in = M->in_dim;
+  const int Mp = pad32(m), in = M->in_dim, in_p = M->in_p;
   MLP_HIP(hipMemsetAsync(M->x_f, 0, (size_t)Mp * in * 4, M->stream));
   MLP_HIP(hipMemcpyAsync(M->x_f, x_host, (size_t)m * in * 4,
                          device_x ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, M->stream));
-  const int nx = Mp * in;
-  hipLaunchKernelGGL(gg_mlp_to_bf16, dim3((nx + 255) / 256), dim3(256), 0, M->stream, M->x_f, M->x_b, nx);
+  const int nx = Mp * in_p;
+  hipLaunchKernelGGL(gg_mlp_to_bf16_rows, dim3((nx + 255) / 256), dim3(256), 0, M->stream, M->x_f, in,
+                     M->x_b, in_p, Mp);
   MLP_HIP(hipGetLastError());
   const uint16_t *h = M->x_b;
-  int hw = in;
+  int hw = in_p;
   for (size_t li = 0; li < M->layers.size(); ++li) {
     const Layer &L = M->layers[li];
     const bool last = li + 1 == M->layers.size();
@@ -266,11 +336,11 @@ static int mlp_backward(gg_mlp *M, int m) {
   for (int li = nl - 1; li >= 0; --li) {
     const Layer &L = M->layers[li];
     const uint16_t *hin = li ? M->Hb[li - 1] : M->x_b;
-    const int win = li ? M->layers[li - 1].out_p : M->in_dim;
+    const int win = li ? M->layers[li - 1].out_p : M->in_p;
     // dW = H_in^T * dZ   ([win x out_p]; only the first L.in rows are real)
     launch_gemm(M->stream, hin, win, 1, M->dAb, L.out_p, 0, M->g + L.w_off, L.out_p, nullptr, 0,
-                nullptr, 0, pad32(L.in), L.out_p, Mp);
-    hipLaunchKernelGGL(gg_mlp_colsum, dim3((L.out_p + 63) / 64), dim3(64), 0, M->stream, M->dA, Mp,
+                nullptr, 0, L.in, L.out_p, Mp);
+    hipLaunchKernelGGL(gg_mlp_colsum, dim3((L.out_p + 63) / 64), dim3(1024), 0, M->stream, M->dA, Mp,
                        L.out_p, M->g + L.b_off);
     if (li == 0) break;
     // dH_in = dZ * W^T   ([Mp x win])
@@ -342,16 +412,13 @@ gg_mlp *gg_mlp_create(int n_layers, const int *dims, uint64_t seed) {
       set_error(GG_ERR_INVALID, "layer widths must be positive");
       return nullptr;
     }
-  if (dims[0] % 32) {
-    set_error(GG_ERR_INVALID, "the input width must be a multiple of 32");
-    return nullptr;
-  }
   if (gg_init(-1) != GG_OK) return nullptr;
   gg_mlp *M = new gg_mlp();
   M->in_dim = dims[0];
+  M->in_p = (dims[0] + 7) & ~7;  // 16-byte BF16 rows for the GEMM's vector loads
   size_t off = 0;
-  int prev_p = dims[0];
-  M->maxw = dims[0];
+  int prev_p = M->in_p;
+  M->maxw = M->in_p;
   for (int i = 0; i < n_layers; ++i) {
     Layer L;
     L.in = prev_p;  // the previous layer's padded width (padding columns stay zero)

@@ -8,8 +8,12 @@
  * evaluator still scores every move that is accepted.
  *
  * Parameter layout (flat FP32 array): for layer i, W_i [rows_i][cols_i]
- * row-major (cols_i = output width rounded up to 32, the extra columns and
- * rows being zero) followed by b_i [cols_i]; see gg_mlp_layout().
+ * row-major (cols_i = output width rounded up to 32; rows_0 = input width
+ * rounded up to 8, rows_i = cols_{i-1}; the extra columns and rows being
+ * zero) followed by b_i [cols_i]; see gg_mlp_layout().
+ *
+ * GEMMs: gfx950 v_mfma_f32_32x32x16_bf16, 64x64 output tiles per 256-thread
+ * workgroup, operands staged through LDS as 16-byte vectors.
  */
 #ifndef GG_MLP_H
 #define GG_MLP_H
@@ -21,7 +25,7 @@ extern "C" {
 
 typedef struct gg_mlp gg_mlp;
 
-/* dims[0..n_layers]: input width (multiple of 32), hidden widths, output width. */
+/* dims[0..n_layers]: input width, hidden widths, output width. */
 gg_mlp *gg_mlp_create(int n_layers, const int *dims, uint64_t seed);
 void gg_mlp_destroy(gg_mlp *m);
 int gg_mlp_param_count(const gg_mlp *m);

@@ -78,3 +78,25 @@ def test_gpu_mlp_trains():
         last = m.train_step(x, t, lr=1e-3)
     assert last < 0.1 * first, (first, last)
     m.close()
+
+
+@pytest.mark.parametrize("dims", [(20, 64, 33, 1), (8, 40, 1), (64, 96, 1)])
+def test_gpu_mlp_odd_shapes_match_reference(dims):
+    """Input widths that are not multiples of 8 (padded to 16-byte BF16 rows
+    inside), hidden widths that are not multiples of 32 or 64 (padded columns,
+    partial 64x64 GEMM tiles): forward and gradient against the reference."""
+    m = Mlp(dims, seed=11)
+    try:
+        rng = np.random.default_rng(5)
+        x = rng.normal(size=(77, dims[0])).astype(np.float32)
+        t = rng.normal(size=77).astype(np.float32)
+        p = m.params()
+        got = m.forward(x)
+        want, _ = OM.forward(p, m.layout, x, bf16=True)
+        assert np.max(np.abs(got - want)) <= 2e-3 * (np.abs(want).max() + 1e-6)
+        loss, g = m.gradient(x, t)
+        l_bf, g_bf = OM.loss_and_grad(p, m.layout, x, t, bf16=True)
+        assert abs(loss - l_bf) <= 1e-3 * abs(l_bf)
+        assert np.max(np.abs(g - g_bf)) <= 2e-2 * np.abs(g_bf).max()
+    finally:
+        m.close()
