@@ -482,15 +482,21 @@ gg_order_rows_kernel(const OrderDesc *__restrict__ descs) {
 
 // Per-graph combination of the unit partial sums
 // (general/evaluation.c:199-200, :241, :247-249 as integers).  One
-// workgroup per graph; its units are contiguous.
+// workgroup per graph; its units are contiguous.  `res` may be pinned host
+// memory (the result then needs no copy).  With `flags` set, the graph's
+// two prep flags (and, in workgroup 0, the queue counter after them) are
+// zeroed for the next run: every kernel that reads them has finished.
 __global__ void __launch_bounds__(256)
 gg_combine_kernel(const GraphMeta *__restrict__ meta, const UnitDesc *__restrict__ units,
-                  const UnitOut *__restrict__ outs, GraphOut *__restrict__ res, int ngraphs) {
+                  const UnitOut *__restrict__ outs, GraphOut *__restrict__ res, int ngraphs,
+                  int32_t *__restrict__ flags) {
   __shared__ uint64_t s64[3][256];
   __shared__ int s32[4][256];
   const int gidx = blockIdx.x;
   const int tid = threadIdx.x;
   if (gidx >= ngraphs) return;
+  if (flags && tid < 2) flags[2 * gidx + tid] = 0;
+  if (flags && gidx == 0 && tid == 2) flags[2 * ngraphs] = 0;
   const GraphMeta m = meta[gidx];
   uint64_t lo = 0, hi = 0, chi = 0;
   int diam = 1, ecc = 0, disc = 0, err = 0;
@@ -554,11 +560,6 @@ gg_combine_kernel(const GraphMeta *__restrict__ meta, const UnitDesc *__restrict
   }
 }
 
-// A tiny fill kernel (captured graphs must not record memset nodes).
-__global__ void gg_clear_i32_kernel(int32_t *p, int n) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0;
-}
 
 // ---------------------------------------------------------------------------
 // Scratch of the allocation-free entry point (gg_apsp_device_noalloc): the
@@ -672,13 +673,15 @@ struct DevBuf {
 struct PinBuf {
   void *p = nullptr;
   size_t cap = 0;
-  int ensure(size_t bytes) {
+  // coherent: kernels write it directly (results), visible to the host once
+  // the stream has been synchronized.
+  int ensure(size_t bytes, bool coherent = false) {
     if (bytes <= cap) return GG_OK;
     if (p) hipHostFree(p);
     p = nullptr;
     cap = 0;
     size_t want = std::max(bytes, (size_t)256);
-    hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+    hipError_t e = hipHostMalloc(&p, want, coherent ? hipHostMallocCoherent : hipHostMallocDefault);
     if (e != hipSuccess)
       return set_error(GG_ERR_HIP, "hipHostMalloc(%zu) failed: %s", want, hipGetErrorString(e));
     cap = want;
@@ -1115,10 +1118,10 @@ int assemble(int count, const gg_graph *gs, int hist_maxd, Assembly &A) {
 
 // Device buffers of one launch set (either the engine's single scratch or a plan's own).
 struct DevSet {
-  DevBuf adj_in, relabel_in, adjT, flags, gdesc, units, outs, meta, gout, hist, prep, order;
-  PinBuf res_h;
+  DevBuf adj_in, relabel_in, adjT, flags, gdesc, units, outs, meta, hist, prep, order;
+  PinBuf res_h;  // per-graph results, written by the combine kernel
   void release() {
-    for (DevBuf *b : {&adj_in, &relabel_in, &adjT, &flags, &gdesc, &units, &outs, &meta, &gout, &hist,
+    for (DevBuf *b : {&adj_in, &relabel_in, &adjT, &flags, &gdesc, &units, &outs, &meta, &hist,
                       &prep, &order})
       b->release();
     res_h.release();
@@ -1135,12 +1138,13 @@ int prepare_buffers(Assembly &A, DevSet &D, int hist_maxd, int32_t *hist_dev,
   // so a one-chunk last row reads 16 bytes past the table.
   if ((rc = D.adjT.ensure(A.adjT_elems * 2 + 64))) return rc;
   if ((rc = D.flags.ensure(((size_t)count * 2 + 1) * 4))) return rc;  // + the queue counter
+  // Zero here; after that every run's combine re-zeroes them.
+  GG_HIP(hipMemsetAsync(D.flags.p, 0, ((size_t)count * 2 + 1) * 4, E.stream));
   if ((rc = D.gdesc.ensure((size_t)count * sizeof(GraphDev)))) return rc;
   if ((rc = D.units.ensure(A.units.size() * sizeof(UnitDesc)))) return rc;
   if ((rc = D.outs.ensure(A.units.size() * sizeof(UnitOut)))) return rc;
   if ((rc = D.meta.ensure((size_t)count * sizeof(GraphMeta)))) return rc;
-  if ((rc = D.gout.ensure((size_t)count * sizeof(GraphOut)))) return rc;
-  if ((rc = D.res_h.ensure((size_t)count * sizeof(GraphOut)))) return rc;
+  if ((rc = D.res_h.ensure((size_t)count * sizeof(GraphOut), true))) return rc;
   for (int i = 0; i < count; ++i) {
     const Slot &s = A.slots[i];
     GraphDev &g = A.gdev[i];
@@ -1287,11 +1291,9 @@ int enqueue(Assembly &A, DevSet &D, const int32_t *const *adj_src, const int32_t
             const int *check_sym, hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr,
             const PatchSrc *patched = nullptr) {
   const int count = (int)A.slots.size();
-  // flags cleared by a kernel-free path: hipMemsetAsync is avoided so the
-  // sequence can be captured (a small fill is folded into the prep launch
-  // ordering below through a dedicated clear kernel).
-  hipLaunchKernelGGL(gg_clear_i32_kernel, dim3((2 * count + 1 + 255) / 256), dim3(256), 0, E.stream,
-                     (int32_t *)D.flags.p, 2 * count + 1);
+  // The flags and the queue counter are zero on entry (zeroed by
+  // prepare_buffers, then by every run's combine), so the sequence needs
+  // neither a memset node nor a fill kernel.
   if (patched) {
     // All slots share one shape (checked at plan creation) and one base table.
     const Slot &s = A.slots[0];
@@ -1338,11 +1340,11 @@ int enqueue(Assembly &A, DevSet &D, const int32_t *const *adj_src, const int32_t
                        lg.unit_begin, lg.n_cap, lg.qcap);
   }
   if (ev_end) GG_HIP(hipEventRecord(ev_end, E.stream));
+  // Results go straight to the pinned host buffer.
   hipLaunchKernelGGL(gg_combine_kernel, dim3(count), dim3(256), 0, E.stream,
                      (const GraphMeta *)D.meta.p, (const UnitDesc *)D.units.p,
-                     (const UnitOut *)D.outs.p, (GraphOut *)D.gout.p, count);
+                     (const UnitOut *)D.outs.p, (GraphOut *)D.res_h.p, count, (int32_t *)D.flags.p);
   GG_HIP(hipGetLastError());
-  GG_HIP(hipMemcpyAsync(D.res_h.p, D.gout.p, count * sizeof(GraphOut), hipMemcpyDeviceToHost, E.stream));
   return GG_OK;
 }
 
@@ -1384,7 +1386,8 @@ static uint64_t *g_trace_host = nullptr;
 // One graph from host memory (the reference ABI's call, repeated with the
 // same shape while the caller changes the table): the assembly, the device
 // descriptors and a captured hipGraph of prep + row order + tile BFS +
-// combine + result copy are kept for the last shape, so a repeated call is
+// combine (results written to pinned host memory) are kept for the last
+// shape, so a repeated call is
 // one adjacency upload, one graph launch and one wait.
 struct SingleCache {
   bool valid = false;
@@ -1640,7 +1643,7 @@ int eval_noalloc(const gg_graph &g, gg_result *out) {
   hipLaunchKernelGGL(k, dim3(units), dim3(block), lds, st, (const GraphDev *)gd_dev,
                      (const UnitDesc *)units_dev, outs_dev, 0, n_cap, qcap);
   hipLaunchKernelGGL(gg_combine_kernel, dim3(1), dim3(256), 0, st, (const GraphMeta *)gm_dev,
-                     (const UnitDesc *)units_dev, (const UnitOut *)outs_dev, gout_dev, 1);
+                     (const UnitDesc *)units_dev, (const UnitOut *)outs_dev, gout_dev, 1, nullptr);
   GG_HIP(hipGetLastError());
   GraphOut r;
   GG_HIP(hipMemcpyAsync(&r, gout_dev, sizeof r, hipMemcpyDeviceToHost, st));
@@ -2074,7 +2077,7 @@ static gg_plan *plan_create(int count, const gg_graph *shapes, int max_patch) {
     set_error(GG_ERR_HIP, "event creation failed");
     return plan_fail(p);
   }
-  // Capture prep + BFS + combine + result copy once; replays reuse it.
+  // Capture prep + BFS + combine once; replays reuse it.
   if (max_patch == 0 && upload_prep(p->A, p->D, p->src.data(), p->rel.data(), p->sym.data()))
     return plan_fail(p);
   if (max_patch > 0 && upload_order(p->A, p->D)) return plan_fail(p);
