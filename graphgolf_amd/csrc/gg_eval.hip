@@ -1672,8 +1672,7 @@ struct Plan {
   std::vector<int> sym;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  hipEvent_t ev_begin = nullptr, ev_end = nullptr;    // around the tile-BFS launches (graph)
-  hipEvent_t tev_begin = nullptr, tev_end = nullptr;  // same, for eager timed launches
+  hipEvent_t tev_begin = nullptr, tev_end = nullptr;  // around the tile-BFS launches (eager, timed)
   bool launched = false, timed = false;
 };
 
@@ -2025,7 +2024,7 @@ int gg_tune_table_load(const char *path) {
 }
 
 static gg_plan *plan_fail(gg_plan *p) {
-  for (hipEvent_t ev : {p->ev_begin, p->ev_end, p->tev_begin, p->tev_end})
+  for (hipEvent_t ev : {p->tev_begin, p->tev_end})
     if (ev) hipEventDestroy(ev);
   p->D.release();
   p->base.release();
@@ -2072,8 +2071,7 @@ static gg_plan *plan_create(int count, const gg_graph *shapes, int max_patch) {
     if (hipStreamSynchronize(engine().stream) != hipSuccess) return plan_fail(p);
     p->psrc = {(const int32_t *)p->base.p, (const int32_t *)p->patches.p, max_patch};
   }
-  if (hipEventCreate(&p->ev_begin) != hipSuccess || hipEventCreate(&p->ev_end) != hipSuccess ||
-      hipEventCreate(&p->tev_begin) != hipSuccess || hipEventCreate(&p->tev_end) != hipSuccess) {
+  if (hipEventCreate(&p->tev_begin) != hipSuccess || hipEventCreate(&p->tev_end) != hipSuccess) {
     set_error(GG_ERR_HIP, "event creation failed");
     return plan_fail(p);
   }
@@ -2087,7 +2085,7 @@ static gg_plan *plan_create(int count, const gg_graph *shapes, int max_patch) {
     set_error(GG_ERR_HIP, "stream capture could not start");
     return plan_fail(p);
   }
-  int rc = enqueue(p->A, p->D, p->src.data(), p->rel.data(), p->sym.data(), p->ev_begin, p->ev_end,
+  int rc = enqueue(p->A, p->D, p->src.data(), p->rel.data(), p->sym.data(), nullptr, nullptr,
                    max_patch > 0 ? &p->psrc : nullptr);
   hipGraph_t graph = nullptr;
   hipError_t e = hipStreamEndCapture(s, &graph);
@@ -2190,10 +2188,13 @@ int gg_plan_launch_timed(gg_plan *p) {
 }
 
 float gg_plan_kernel_ms(gg_plan *p) {
-  if (!p) return -1.0f;
+  // Only eager timed launches carry events (the captured graph has none).
+  if (!p || !p->timed || !p->tev_begin) return -1.0f;
   float ms = -1.0f;
-  hipEvent_t b = p->timed ? p->tev_begin : p->ev_begin, e = p->timed ? p->tev_end : p->ev_end;
-  if (!b || hipEventElapsedTime(&ms, b, e) != hipSuccess) return -1.0f;
+  if (hipEventElapsedTime(&ms, p->tev_begin, p->tev_end) != hipSuccess) {
+    (void)hipGetLastError();  // leave no error behind for the next launch's check
+    return -1.0f;
+  }
   return ms;
 }
 
@@ -2221,7 +2222,7 @@ void gg_plan_destroy(gg_plan *p) {
   if (p->graph) hipGraphDestroy(p->graph);
   p->base.release();
   p->patches.release();
-  for (hipEvent_t ev : {p->ev_begin, p->ev_end, p->tev_begin, p->tev_end})
+  for (hipEvent_t ev : {p->tev_begin, p->tev_end})
     if (ev) hipEventDestroy(ev);
   p->D.release();
   delete p;

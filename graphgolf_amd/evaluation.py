@@ -155,7 +155,8 @@ class Plan:
         return [_result(self._out[i], self.nodes[i]) for i in range(self.n)]
 
     def kernel_ms(self) -> float:
-        """Tile-BFS kernel time of the last completed replay (HIP events)."""
+        """Tile-BFS kernel time (ms, HIP events) of the last completed
+        launch_timed(); -1 after run()/launch(), whose graph carries no events."""
         return float(L.lib().gg_plan_kernel_ms(self._p))
 
     def config(self) -> dict:

@@ -139,7 +139,8 @@ int gg_plan_set_patches(gg_plan *p, const int32_t *patches_host);
  * events around the tile-BFS launches; complete it with gg_plan_wait(). */
 int gg_plan_launch_timed(gg_plan *p);
 /* Duration (ms, HIP events on the evaluator's stream) of the tile-BFS
- * kernels in the plan's last completed launch; -1 if unavailable. */
+ * kernels in the plan's last completed gg_plan_launch_timed(); -1 after a
+ * graph launch (gg_plan_launch / gg_plan_run carry no events). */
 float gg_plan_kernel_ms(gg_plan *p);
 /* Waits for all work of the evaluator's stream and the device. */
 int gg_synchronize(void);

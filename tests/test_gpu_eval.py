@@ -261,6 +261,42 @@ def test_gpu_plan_replays():
     plan.close()
 
 
+def test_gpu_plan_timed_and_graph_launches_interleave():
+    """Graph replays and eager timed launches of one plan alternate: kernel
+    times exist only after timed launches (and reading them after a replay
+    leaves no error for the next launch); a replay that meets an
+    out-of-range entry reports it, and the next replay of a valid table is
+    clean (the prep flags start from zero every run)."""
+    n, d = 2048, 12
+    a = G.edges_to_adj(n, d, G.random_regular(n, d, 8))
+    want = oracle.matrix_op(a)
+    plan = E.Plan([dict(nodes=n, degree=d, symmetric=True)])
+    plan.set_adj(0, a)
+    for rep in range(3):
+        r = plan.run()[0]
+        assert (r.diameter, r.sum) == (want.diameter, want.S)
+        assert plan.kernel_ms() == -1.0
+        plan.launch_timed()
+        r = plan.wait()[0]
+        assert (r.diameter, r.sum) == (want.diameter, want.S)
+        assert plan.kernel_ms() > 0.0
+    bad = a.copy()
+    bad[7, 3] = n
+    plan.set_adj(0, bad)
+    with pytest.raises(L.GGError) as e:
+        plan.run()
+    assert e.value.code == L.GG_ERR_INVALID
+    plan.set_adj(0, a)
+    for launch in ("graph", "timed"):
+        if launch == "graph":
+            r = plan.run()[0]
+        else:
+            plan.launch_timed()
+            r = plan.wait()[0]
+        assert (r.connected, r.diameter, r.sum) == (True, want.diameter, want.S)
+    plan.close()
+
+
 def test_gpu_c5_full_size():
     """n=65536 d=16: golden vector (reference evaluator), plus size-independent
     properties: word-width invariance and relabelling invariance."""
