@@ -1116,6 +1116,9 @@ int assemble(int count, const gg_graph *gs, int hist_maxd, Assembly &A) {
   return GG_OK;
 }
 
+bool order_applies_mode(int mode, int size, int min_size, int dp);
+constexpr int kOrderMinTiles = 1024;
+
 // Device buffers of one launch set (either the engine's single scratch or a plan's own).
 struct DevSet {
   DevBuf adj_in, relabel_in, adjT, flags, gdesc, units, outs, meta, hist, prep, order;
@@ -1152,6 +1155,10 @@ int prepare_buffers(Assembly &A, DevSet &D, int hist_maxd, int32_t *hist_dev,
     g.flags = (int32_t *)D.flags.p + 2 * i;
     g.n = s.n;
     g.dp = s.dp;
+    // The device row order permutes padding into the rows: no skipping then.
+    g.pad4 = s.d <= s.dp - 4 &&
+                     !order_applies_mode(A.order_mode, A.meta[i].unit_count, kOrderMinTiles, s.dp)
+                 ? 1 : 0;
     g.split = s.b * s.g;
     g.c = s.c;
     g.push_ok = 1;
@@ -1204,7 +1211,6 @@ int upload_prep(Assembly &A, DevSet &D, const int32_t *const *adj_src, const int
 // (measured: 3.5 cycles per wave-read spread over the 32 slots, 6.7 at
 // random), so their key is the id itself, as for 32-bit words.
 int g_row_order = 1;
-constexpr int kOrderMinTiles = 1024;
 constexpr int kOrderMinNodes = 8192;
 
 int order_shift(int bits) { return bits == 8 ? 0 : bits == 16 ? 1 : 2; }

@@ -34,6 +34,9 @@ struct GraphDev {
   // some vertex is unreached).
   uint32_t *src_sum = nullptr;
   uint16_t *src_ecc = nullptr;
+  // 1: the last 4 entries of every row are padding (degree <= dp - 4); the
+  // pull levels of one-chunk rows then skip them.
+  int32_t pad4 = 0;
 };
 
 // Kernel modes.

@@ -11,6 +11,8 @@ Evaluations with the order on and off must agree exactly.
 import numpy as np
 import pytest
 
+import oracle
+
 from graphgolf_amd import evaluation as E
 from graphgolf_amd import graphs as G
 
@@ -126,6 +128,29 @@ def test_small_graphs_same_with_order_everywhere(n, d, bits):
     finally:
         E.set_word_bits(0)
     assert (on.diameter, on.sum, on.word_bits) == (off.diameter, off.sum, off.word_bits)
+
+
+@pytest.mark.parametrize("bits", [8, 16, 32, 64])
+@pytest.mark.parametrize("n,d", [(2000, 3), (3000, 4), (1500, 4)])
+def test_padded_one_chunk_rows_with_and_without_order(n, d, bits):
+    """Rows of at most 4 entries carry 4 or more padding entries (the row's
+    own id) in their one 16-byte chunk: without the device row order the
+    pull levels skip the last 4; with it, padding is permuted into the row
+    and every entry is gathered.  Both agree with the oracle."""
+    E.init(0)
+    adj = G.edges_to_adj(n, d, G.random_regular(n, d, 17))
+    want = oracle.matrix_op(adj)
+    E.set_word_bits(bits)
+    try:
+        for mode in (0, 2):
+            prev = E.set_row_order(mode)
+            try:
+                got = E.apsp(adj, n, 1)
+            finally:
+                E.set_row_order(prev)
+            assert (got.connected, got.diameter, got.sum) == (want.ok, want.diameter, want.S)
+    finally:
+        E.set_word_bits(0)
 
 
 def test_batch_same_with_order_everywhere():
