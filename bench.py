@@ -3,8 +3,12 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c5]
 
 A step is one exact evaluation (diameter + distance sum S) of the workload
-graph(s), with the adjacency already resident in HBM: one replay of a captured
-hipGraph (adjacency transposition + tile BFS + per-graph combine + result copy).
+graph(s), with the adjacency already resident in HBM: the plan's kernels
+(adjacency transposition + tile BFS + per-graph combine, which writes the
+results to pinned host memory) enqueued on the evaluator's stream with HIP
+events around the tile BFS, then a wait for the results.  (The plan's
+captured hipGraph holds the same sequence; replaying it measured 2-4 us
+slower per step.)
 A single general graph is first numbered by the bank-balanced layout
 (gg_bank_layout, once, before timing; `config.layout` reports its time and
 bank cost): an isomorphic numbering with each row's entries in slot order,
