@@ -13,6 +13,7 @@
 // accepted move is still scored by the exact evaluator.
 
 #include <hip/hip_runtime.h>
+#include <cassert>
 
 #include <cmath>
 #include <cstdint>
@@ -66,7 +67,10 @@ __global__ void gg_mlp_to_bf16_rows(const float *__restrict__ in, int cols, uint
 // every lane reads its 8 consecutive k of one row or column with one
 // 16-byte LDS read.  Operand layout of the MFMA: lane l holds A(l%32,
 // 8*(l/32)+e) and B(8*(l/32)+e, l%32), e = 0..7; accumulator register r of
-// lane l is D(8*(r/4) + 4*(l/32) + r%4, l%32).
+// lane l is D(8*(r/4) + 4*(l/32) + r%4, l%32).  Shapes the callers keep:
+// N a multiple of 32 (a wave's quarter is whole or wholly past N), K a
+// multiple of 8 and, for a transposed A, M a multiple of 8 (each 16-byte
+// load is whole or wholly out of range); rows past M are not stored.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 constexpr int kGemmTile = 64, kGemmK = 32, kGemmLd = kGemmK + 8;
@@ -292,6 +296,8 @@ static int mlp_reserve(gg_mlp *M, int m) {
 static void launch_gemm(hipStream_t s, const uint16_t *A, int lda, int tA, const uint16_t *B, int ldb,
                         int tB, float *C, int ldc, uint16_t *Cb, int ldh, const float *bias, int relu,
                         int Mr, int Nr, int K) {
+  // Widths are padded at creation (outputs to 32, the input to 8).
+  assert(Nr % 32 == 0 && K % 8 == 0 && (!tA || Mr % 8 == 0));
   hipLaunchKernelGGL(gg_mlp_gemm, dim3((Mr + kGemmTile - 1) / kGemmTile, (Nr + kGemmTile - 1) / kGemmTile),
                      dim3(256), 0, s, A, lda, tA, B, ldb, tB, C, ldc, Cb, ldh, bias, relu, Mr, Nr, K);
 }
