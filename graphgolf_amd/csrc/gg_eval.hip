@@ -653,7 +653,7 @@ struct DevBuf {
   size_t cap = 0;
   int ensure(size_t bytes) {
     if (bytes <= cap) return GG_OK;
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
     p = nullptr;
     cap = 0;
     size_t want = std::max(bytes, (size_t)256);
@@ -664,7 +664,7 @@ struct DevBuf {
     return GG_OK;
   }
   void release() {
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
     p = nullptr;
     cap = 0;
   }
@@ -677,7 +677,7 @@ struct PinBuf {
   // the stream has been synchronized.
   int ensure(size_t bytes, bool coherent = false) {
     if (bytes <= cap) return GG_OK;
-    if (p) hipHostFree(p);
+    if (p) (void)hipHostFree(p);
     p = nullptr;
     cap = 0;
     size_t want = std::max(bytes, (size_t)256);
@@ -688,7 +688,7 @@ struct PinBuf {
     return GG_OK;
   }
   void release() {
-    if (p) hipHostFree(p);
+    if (p) (void)hipHostFree(p);
     p = nullptr;
     cap = 0;
   }
@@ -907,8 +907,8 @@ int init(int device) {
 
 void finalize() {
   if (!E.ready) return;
-  hipStreamSynchronize(E.stream);
-  hipStreamDestroy(E.stream);
+  (void)hipStreamSynchronize(E.stream);
+  (void)hipStreamDestroy(E.stream);
   E.stream = nullptr;
   E.ready = false;
 }
@@ -1406,8 +1406,8 @@ struct SingleCache {
   hipGraphExec_t exec = nullptr;
   PinBuf stage;  // pinned staging of the table's upload
   void drop() {
-    if (exec) hipGraphExecDestroy(exec);
-    if (graph) hipGraphDestroy(graph);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
     exec = nullptr;
     graph = nullptr;
     valid = false;
@@ -1445,7 +1445,7 @@ int eval_single_cached(const gg_graph &g, gg_result *out) {
     const hipError_t e = hipStreamEndCapture(E.stream, &graph);
     if (rc || e != hipSuccess || !graph ||
         hipGraphInstantiate(&C.exec, graph, nullptr, nullptr, 0) != hipSuccess) {
-      if (graph) hipGraphDestroy(graph);
+      if (graph) (void)hipGraphDestroy(graph);
       C.exec = nullptr;
       return rc ? rc : set_error(GG_ERR_HIP, "single-graph capture failed: %s", hipGetErrorString(e));
     }
@@ -1907,9 +1907,10 @@ static int tune_run(int count, const gg_graph *graphs, int reps, Config *best_cf
     for (int i = 0; i < count && rc == GG_OK; ++i) rc = gg_plan_set_adj(p, i, graphs[i].adj);
     if (rc == GG_OK) rc = gg_plan_run(p, res.data());  // warm-up
     if (rc == GG_OK) {
-      hipEventRecord(e0, E.stream);
+      if (hipEventRecord(e0, E.stream) != hipSuccess) rc = set_error(GG_ERR_HIP, "tuner event record failed");
       for (int r = 0; r < reps && rc == GG_OK; ++r) rc = gg_plan_launch(p);
-      hipEventRecord(e1, E.stream);
+      if (rc == GG_OK && hipEventRecord(e1, E.stream) != hipSuccess)
+        rc = set_error(GG_ERR_HIP, "tuner event record failed");
       if (rc == GG_OK) rc = gg_plan_wait(p, res.data());
     }
     float ms = 0;
@@ -1937,8 +1938,8 @@ static int tune_run(int count, const gg_graph *graphs, int reps, Config *best_cf
       best = cands[k];
     }
   }
-  if (e0) hipEventDestroy(e0);
-  if (e1) hipEventDestroy(e1);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
   E.forced_bits = saved_bits;
   if (rc) return rc;
   const gg_graph &g = graphs[big];
@@ -2033,7 +2034,7 @@ int gg_tune_table_load(const char *path) {
 
 static gg_plan *plan_fail(gg_plan *p) {
   for (hipEvent_t ev : {p->tev_begin, p->tev_end})
-    if (ev) hipEventDestroy(ev);
+    if (ev) (void)hipEventDestroy(ev);
   p->D.release();
   p->base.release();
   p->patches.release();
@@ -2100,7 +2101,7 @@ static gg_plan *plan_create(int count, const gg_graph *shapes, int max_patch) {
   if (rc || e != hipSuccess || !graph ||
       hipGraphInstantiate(&p->exec, graph, nullptr, nullptr, 0) != hipSuccess) {
     if (!rc) set_error(GG_ERR_HIP, "graph capture/instantiate failed: %s", hipGetErrorString(e));
-    if (graph) hipGraphDestroy(graph);
+    if (graph) (void)hipGraphDestroy(graph);
     return plan_fail(p);
   }
   p->graph = graph;
@@ -2225,13 +2226,13 @@ int gg_synchronize(void) {
 
 void gg_plan_destroy(gg_plan *p) {
   if (!p) return;
-  hipStreamSynchronize(engine().stream);
-  if (p->exec) hipGraphExecDestroy(p->exec);
-  if (p->graph) hipGraphDestroy(p->graph);
+  (void)hipStreamSynchronize(engine().stream);
+  if (p->exec) (void)hipGraphExecDestroy(p->exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
   p->base.release();
   p->patches.release();
   for (hipEvent_t ev : {p->tev_begin, p->tev_end})
-    if (ev) hipEventDestroy(ev);
+    if (ev) (void)hipEventDestroy(ev);
   p->D.release();
   delete p;
 }

@@ -345,10 +345,10 @@ int free_all(gg_incr *s) {
                   s->aff_count, s->aff_sum_old, s->hist, s->removed_hist, s->src_sum,
                   s->new_sum, s->src_ecc, s->new_ecc, s->outs, s->res};
   for (void *p : ptrs)
-    if (p) hipFree(p);
-  if (s->res_h) hipHostFree(s->res_h);
-  if (s->probe_src_h) hipHostFree(s->probe_src_h);
-  if (s->pdesc_h) hipHostFree(s->pdesc_h);
+    if (p) (void)hipFree(p);
+  if (s->res_h) (void)hipHostFree(s->res_h);
+  if (s->probe_src_h) (void)hipHostFree(s->probe_src_h);
+  if (s->pdesc_h) (void)hipHostFree(s->pdesc_h);
   return GG_OK;
 }
 
@@ -735,7 +735,7 @@ int gg_incr_adjacency(const gg_incr *s, int32_t *adj) {
 
 void gg_incr_destroy(gg_incr *s) {
   if (!s) return;
-  hipStreamSynchronize(s->stream);
+  (void)hipStreamSynchronize(s->stream);
   free_all(s);
   delete s;
 }

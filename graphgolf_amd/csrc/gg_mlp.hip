@@ -262,17 +262,17 @@ struct gg_mlp {
 static int mlp_reserve(gg_mlp *M, int m) {
   const int Mp = pad32(m);
   if (Mp <= M->cap_m) return GG_OK;
-  for (float *z : M->Z) hipFree(z);
-  for (uint16_t *h : M->Hb) hipFree(h);
+  // Grow: release the row buffers (pointers cleared, so that a failed
+  // allocation below leaves nothing for gg_mlp_destroy to free twice).
+  for (float *z : M->Z) (void)hipFree(z);
+  for (uint16_t *h : M->Hb) (void)hipFree(h);
   M->Z.clear();
   M->Hb.clear();
-  hipFree(M->x_b);
-  hipFree(M->dA);
-  hipFree(M->dB);
-  hipFree(M->dAb);
-  hipFree(M->x_f);
-  hipFree(M->t_f);
-  hipFree(M->y_f);
+  for (void **q : {(void **)&M->x_b, (void **)&M->dA, (void **)&M->dB, (void **)&M->dAb,
+                   (void **)&M->x_f, (void **)&M->t_f, (void **)&M->y_f}) {
+    if (*q) (void)hipFree(*q);
+    *q = nullptr;
+  }
   M->cap_m = 0;
   MLP_HIP(hipMalloc(&M->x_b, (size_t)Mp * M->in_p * 2));
   MLP_HIP(hipMalloc(&M->x_f, (size_t)Mp * M->in_dim * 4));
@@ -458,8 +458,8 @@ gg_mlp *gg_mlp_create(int n_layers, const int *dims, uint64_t seed) {
       for (int c = 0; c < L.out; ++c) host[L.w_off + (size_t)r * L.out_p + c] = (float)((2 * rnd() - 1) * lim);
     real_in = L.out;
   }
-  hipStreamCreateWithFlags(&M->stream, hipStreamNonBlocking);
-  bool ok = hipMalloc(&M->p, off * 4) == hipSuccess && hipMalloc(&M->g, off * 4) == hipSuccess &&
+  bool ok = hipStreamCreateWithFlags(&M->stream, hipStreamNonBlocking) == hipSuccess &&
+            hipMalloc(&M->p, off * 4) == hipSuccess && hipMalloc(&M->g, off * 4) == hipSuccess &&
             hipMalloc(&M->m1, off * 4) == hipSuccess && hipMalloc(&M->m2, off * 4) == hipSuccess &&
             hipMalloc(&M->pb, off * 2) == hipSuccess && hipMalloc(&M->loss, 4) == hipSuccess;
   if (!ok) {
@@ -467,10 +467,13 @@ gg_mlp *gg_mlp_create(int n_layers, const int *dims, uint64_t seed) {
     gg_mlp_destroy(M);
     return nullptr;
   }
-  hipMemcpy(M->p, host.data(), off * 4, hipMemcpyHostToDevice);
-  hipMemset(M->m1, 0, off * 4);
-  hipMemset(M->m2, 0, off * 4);
-  hipMemset(M->g, 0, off * 4);
+  if (hipMemcpy(M->p, host.data(), off * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(M->m1, 0, off * 4) != hipSuccess || hipMemset(M->m2, 0, off * 4) != hipSuccess ||
+      hipMemset(M->g, 0, off * 4) != hipSuccess) {
+    set_error(GG_ERR_HIP, "MLP parameter upload failed");
+    gg_mlp_destroy(M);
+    return nullptr;
+  }
   if (refresh_bf16(M) || hipStreamSynchronize(M->stream) != hipSuccess) {
     gg_mlp_destroy(M);
     return nullptr;
@@ -480,14 +483,15 @@ gg_mlp *gg_mlp_create(int n_layers, const int *dims, uint64_t seed) {
 
 void gg_mlp_destroy(gg_mlp *M) {
   if (!M) return;
-  if (M->stream) hipStreamSynchronize(M->stream);
-  for (float *z : M->Z) hipFree(z);
-  for (uint16_t *h : M->Hb) hipFree(h);
+  // Release path: errors are not reported.
+  if (M->stream) (void)hipStreamSynchronize(M->stream);
+  for (float *z : M->Z) (void)hipFree(z);
+  for (uint16_t *h : M->Hb) (void)hipFree(h);
   for (void *q : {(void *)M->p, (void *)M->g, (void *)M->m1, (void *)M->m2, (void *)M->pb,
                   (void *)M->x_b, (void *)M->dA, (void *)M->dB, (void *)M->dAb, (void *)M->x_f,
                   (void *)M->t_f, (void *)M->loss, (void *)M->y_f})
-    if (q) hipFree(q);
-  if (M->stream) hipStreamDestroy(M->stream);
+    if (q) (void)hipFree(q);
+  if (M->stream) (void)hipStreamDestroy(M->stream);
   delete M;
 }
 
