@@ -353,8 +353,8 @@ gg_order_rows_kernel(const OrderDesc *__restrict__ descs) {
 
   for (int s = 0; s < DP; ++s) {
     const uint32_t cap = (uint32_t)(clo + (s < kh ? 1 : 0));
-    // Round 1: every bank has room; the entry on the bank with the most
-    // unused entries (key = degree << 5 | entry).
+    // First ticket pass: every bank has room; the entry on the bank with
+    // the most unused entries (key = degree << 5 | entry).
     uint32_t kd[DP];
     uint32_t best = 0;
 #pragma unroll
@@ -368,8 +368,8 @@ gg_order_rows_kernel(const OrderDesc *__restrict__ descs) {
     bool placed = !live;
     if (live) placed = (atomicAdd(&S.u.tk[s][h][bw & 31u], kTicket) & 255u) < cap;
     wsync();
-    // Later rounds for the rest: the bank with room and the most unused
-    // entries, else (and in the last round always) the least-loaded bank.
+    // Later ticket passes for the rest: the bank with room and the most
+    // unused entries, else (and in the last pass always) the least-loaded bank.
     for (int round = 1; round <= kOrderRounds; ++round) {
       if (__builtin_amdgcn_ballot_w64(!placed) == 0ull) break;
       if (!placed) {
